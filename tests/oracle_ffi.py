@@ -1,0 +1,128 @@
+"""ctypes bindings of the CPU oracle (oracle/liborc.so) -- test infrastructure."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i64, i32, f32 = C.c_int64, C.c_int, C.c_float
+f32p_or_null = C.c_void_p
+
+
+def _sig(lib, name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(ROOT, "oracle", "liborc.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle/liborc.so is not built (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    _sig(lib, "orc_morton3", C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64)
+    _sig(lib, "orc_color_buckets", None, i64, i64, i64, i32, i64p, i64, i64p, i64p)
+    _sig(lib, "orc_halo_fill", None, i64, i64, i64, i32, f32p, i32p, i64, i32, i32, i32, f32p)
+    _sig(lib, "orc_halo_fill_multires", None, i64, i64, i64, i32, f32p, i32p, f32p, i32p, u8p, i64, i32, i32,
+         i32, f32p)
+    for k in ("laplacian", "meancurv", "bilaplacian"):
+        _sig(lib, "orc_kernel_" + k, None, i32, f32p, f32, i32, f32p)
+    _sig(lib, "orc_sweep", None, i64, i64, i64, f32p, i32p, i64p, i64, i32, f32, i32, i32, i32, i32)
+    _sig(lib, "orc_sweep_multires", None, i64, i64, i64, i32, f32p, i32p, f32p, i32p, u8p, i64p, i64, i32, f32,
+         i32, i32, i32)
+    _sig(lib, "orc_downsample_avg", None, i32, f32p, i32p, f32p, i32p, i64p, i64)
+    _sig(lib, "orc_sample_coarse", f32, i64, i64, i64, i32, f32p, i32p, i32, i32, i32)
+    _sig(lib, "orc_regularize", None, u8p, i64, i64, i64, i32)
+    _sig(lib, "orc_active_blocks", i64, i64, i64, i64, f32p, i64, f32, i64p)
+    _sig(lib, "orc_bucket_indices", i64, i64p, i64, i64, i64p, i64p, i64p)
+    _sig(lib, "orc_particle_bids", None, i64, i64, i64, f32p, i64, i64p)
+    _sig(lib, "orc_morton_keys", None, f32p, i64, u32p)
+    _sig(lib, "orc_stable_sort_perm", None, u32p, i64, u32p)
+    _sig(lib, "orc_sample", None, i64, i64, i64, f32p, i32p, f32p, i64, i32, i32, i32, f32p_or_null,
+         f32p_or_null, f32p_or_null)
+    _sig(lib, "orc_mac_sample", None, i64, i64, i64, f32p, i64, i32p, f32p, i64, i32, i32, f32p)
+    _sig(lib, "orc_g2p_blend", None, f32p, f32p, f32p, i64, f32, f32p)
+    _sig(lib, "orc_p2g_mac", None, i64, i64, i64, f32p, f32p, f32p, i64p, i64p, i64, f32, f32p, i64, i32p,
+         i64p, i64, i32)
+    _sig(lib, "orc_p2g_mac_naive", None, i64, i64, i64, f32p, f32p, f32p, i64, f32, f32p)
+    _LIB = Oracle(lib)
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Oracle:
+    """Thin numpy-friendly wrappers over liborc.so."""
+
+    def __init__(self, lib):
+        self.lib = lib
+
+    def __getattr__(self, name):
+        return getattr(self.lib, name)
+
+    def halo_fill(self, g, bid, halo, full, bc=1):
+        hsx = g.B + 2 * halo
+        tile = np.zeros(hsx ** 3, np.float32)
+        self.lib.orc_halo_fill(g.sx, g.sy, g.sz, g.L, np.ascontiguousarray(g.slab), g.b2s, bid, halo, int(full),
+                               bc, tile)
+        return tile.reshape(hsx, hsx, hsx)
+
+    def kernel(self, stencil, tile, dt, clamp01=False, B=16):
+        out = np.zeros(B ** 3, np.float32)
+        name = {0: "laplacian", 1: "meancurv", 2: "bilaplacian"}[stencil]
+        getattr(self.lib, "orc_kernel_" + name)(B, np.ascontiguousarray(tile, np.float32).ravel(), dt,
+                                                 int(clamp01), out)
+        return out
+
+    def sweep(self, g, active, stencil, dt, iterations, clamp01=False, nthreads=1, ftz=False):
+        """In place over g's slab (Sweeper::sweep)."""
+        slab = np.ascontiguousarray(g.slab)
+        act = np.ascontiguousarray(active, np.int64)
+        self.lib.orc_sweep(g.sx, g.sy, g.sz, slab, g.b2s, act, act.size, stencil, dt, iterations, int(clamp01),
+                           nthreads, int(ftz))
+        g._slab[: g.n_slots] = slab
+
+    def color_buckets(self, g, active):
+        act = np.ascontiguousarray(active, np.int64)
+        out = np.zeros(max(act.size, 1), np.int64)
+        counts = np.zeros(8, np.int64)
+        self.lib.orc_color_buckets(g.sx, g.sy, g.sz, g.L, act, act.size, out, counts)
+        return out[: act.size], counts
+
+    def sample(self, g, pos, interp, align, bc, want_grad=False, want_hess=False):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        val = np.zeros(n, np.float32)
+        grad = np.zeros((n, 3), np.float32) if want_grad else None
+        hess = np.zeros((n, 6), np.float32) if want_hess else None
+        self.lib.orc_sample(g.sx, g.sy, g.sz, np.ascontiguousarray(g.slab), g.b2s, pos, n, interp, align, bc,
+                            _ptr(val), _ptr(grad), _ptr(hess))
+        return val, grad, hess
+
+    def active_blocks(self, sx, sy, sz, pos, r_scan):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nb = ((sx + 15) >> 4) * ((sy + 15) >> 4) * ((sz + 15) >> 4)
+        out = np.zeros(nb, np.int64)
+        k = self.lib.orc_active_blocks(sx, sy, sz, pos, pos.shape[0], r_scan, out)
+        return out[:k]
+
+    def bucket_indices(self, bids, n_blocks):
+        bids = np.ascontiguousarray(bids, np.int64)
+        n = bids.size
+        pb = np.zeros(max(n, 1), np.int64)
+        starts = np.zeros(n + 1, np.int64)
+        perm = np.zeros(max(n, 1), np.int64)
+        k = self.lib.orc_bucket_indices(bids, n, n_blocks, pb, starts, perm)
+        return pb[:k], starts[: k + 1], perm[:n]
