@@ -1,0 +1,147 @@
+/*
+ * msbg_hip.h -- C ABI of the MI355X (gfx950) sparse-block-grid smoother and
+ * particle-transfer path.
+ *
+ * Every entry point returns an int status (0 = ok, < 0 = error; the message
+ * is available from msbg_last_error()).  No entry point unwinds or aborts;
+ * the preconditions the reference only debug-asserts are checked here.
+ * Pointers documented as "host" are read or written synchronously and not
+ * retained; pointers documented as "device" must be device memory (from
+ * msbg_malloc, hipMalloc or a torch tensor on the same device).  All work runs
+ * on the library stream (msbg_set_stream, default the null stream); calls are
+ * synchronous unless their name ends in _async.
+ *
+ * Reference interfaces are cited as <file>:<line> of msbg-rs.
+ *
+ * Layout of a grid level with block edge B = 2^L (L = 4 for 16^3 blocks):
+ *   nx = ceil(sx / B) (sparse_grid.rs:131-136), bid = bx + by*nx + bz*nx*ny,
+ *   vid = vx | vy << L | vz << 2L (sparse_grid.rs:159-175);
+ *   b2s[bid] : 0 = empty dummy (absent block), 1 = full dummy, >= 2 = slot of
+ *              an allocated value block;
+ *   slab[slot * B^3 + vid] : payload; slots 0 / 1 hold the empty / full value.
+ */
+#ifndef MSBG_HIP_H
+#define MSBG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSBG_OK = 0,
+    MSBG_EINVAL = -1, /* invalid argument / violated precondition */
+    MSBG_ENOMEM = -2, /* block capacity or device memory exhausted */
+    MSBG_EHIP = -3,   /* HIP runtime error */
+};
+
+/* Stencil -- solver.rs:43-51. */
+enum { MSBG_LAPLACIAN = 0, MSBG_MEAN_CURVATURE = 1, MSBG_BILAPLACIAN = 2 };
+/* BoundaryCondition -- math/boundary.rs:6-14. */
+enum { MSBG_BC_CLAMP = 0, MSBG_BC_NEUMANN = 1, MSBG_BC_DIRICHLET = 2 };
+
+/* ---- runtime -------------------------------------------------------------- */
+const char *msbg_last_error(void);
+int msbg_device_count(int *n);
+int msbg_set_stream(void *hip_stream); /* NULL = default stream */
+int msbg_synchronize(void);
+int msbg_malloc(void **dev_ptr, size_t bytes);
+int msbg_free(void *dev_ptr);
+int msbg_memcpy_htod(void *dev_dst, const void *host_src, size_t bytes);
+int msbg_memcpy_dtoh(void *host_dst, const void *dev_src, size_t bytes);
+int msbg_memcpy_dtod(void *dev_dst, const void *dev_src, size_t bytes);
+/* Timing with HIP events on the library stream. */
+int msbg_event_create(void **ev);
+int msbg_event_destroy(void *ev);
+int msbg_event_record(void *ev);
+int msbg_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- sparse grid -----------------------------------------------------------
+ * A device-resident SparseGrid<f32, B, B^3> (sparse_grid.rs:61-155) or one
+ * multires LevelData level (multires/level.rs:84-243).  The block pool is a
+ * fixed-capacity slab (blockpool.rs:73-195): allocations beyond
+ * `capacity_blocks` fail with MSBG_ENOMEM, as try_alloc_block returns None
+ * (blockpool.rs:118-131). */
+typedef struct msbg_grid msbg_grid;
+
+/* SparseGrid::new -- sparse_grid.rs:107-155 (log2_bsx 1..4). */
+int msbg_grid_new(int64_t sx, int64_t sy, int64_t sz, int log2_bsx, float empty_value, float full_value,
+                  int64_t capacity_blocks, msbg_grid **out);
+int msbg_grid_free(msbg_grid *g);
+/* dims[0..7] = sx, sy, sz, nx, ny, nz, n_blocks, n_slots (host). */
+int msbg_grid_dims(const msbg_grid *g, int64_t *dims);
+/* ensure_blocks_parallel -- sparse_grid.rs:333-355 (host bids, unique). */
+int msbg_grid_ensure_blocks(msbg_grid *g, const int64_t *bids, int64_t n);
+/* fill_blocks_parallel -- sparse_grid.rs:359-369 (host bids). */
+int msbg_grid_fill_blocks(msbg_grid *g, const int64_t *bids, int64_t n, float value);
+/* set_full_block / set_empty_block -- sparse_grid.rs:270-282. */
+int msbg_grid_set_full_block(msbg_grid *g, int64_t bid);
+int msbg_grid_set_empty_block(msbg_grid *g, int64_t bid);
+/* set_empty_value / set_full_value -- sparse_grid.rs:253-266. */
+int msbg_grid_set_empty_value(msbg_grid *g, float v);
+int msbg_grid_set_full_value(msbg_grid *g, float v);
+/* Payload upload / download for `bids` (host, n * B^3 floats).  Reading a
+ * dummy or absent block returns its sentinel value, as get_value_in_block
+ * (sparse_grid.rs:301-308); writing one is MSBG_EINVAL. */
+int msbg_grid_write_blocks(msbg_grid *g, const int64_t *bids, int64_t n, const float *host_data);
+int msbg_grid_read_blocks(const msbg_grid *g, const int64_t *bids, int64_t n, float *host_out);
+/* Bulk load / store of the whole layout (host b2s[n_blocks], slab[n_slots*B^3]). */
+int msbg_grid_load(msbg_grid *g, const int32_t *b2s, const float *slab, int64_t n_slots);
+int msbg_grid_store(const msbg_grid *g, int32_t *b2s, float *slab, int64_t cap_slots, int64_t *n_slots);
+/* value_block_ptr_mut -- sparse_grid.rs:424-429: device payload, NULL for dummies. */
+int msbg_grid_value_block_ptr(msbg_grid *g, int64_t bid, float **dev_ptr);
+/* Device views of the slab and the bid->slot table. */
+int msbg_grid_device_views(msbg_grid *g, float **slab, int32_t **b2s);
+
+/* ---- colour schedule and sweeps -------------------------------------------- */
+/* build_color_buckets -- solver.rs:247-267: the 8 buckets back to back in
+ * `out` (host, n entries), sizes in counts[8]; each bucket Morton-sorted. */
+int msbg_color_buckets(const msbg_grid *g, const int64_t *active, int64_t n, int64_t *out, int64_t *counts);
+
+/* Sweeper::sweep -- solver.rs:166-217 (with apply_pde, solver.rs:228-240):
+ * `iterations` 8-colour in-place passes over `active` (host bids) with the
+ * Neumann halo; dummy / absent blocks are skipped.  Results are bit-identical
+ * to the reference's arithmetic (fma where it uses mul_add). */
+int msbg_sweep(msbg_grid *g, const int64_t *active, int64_t n, int stencil, float dt, int iterations,
+               int clamp01);
+
+/* A reusable sweep plan: the colour buckets of `active` resident on the
+ * device, run many times without host work (Sweeper::new + repeated sweep). */
+typedef struct msbg_sweep_plan msbg_sweep_plan;
+int msbg_sweep_plan_new(msbg_grid *g, const int64_t *active, int64_t n, msbg_sweep_plan **out);
+int msbg_sweep_plan_free(msbg_sweep_plan *p);
+int msbg_sweep_plan_run_async(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01);
+
+/* HaloBlock::fill::<HALO, FULL> -- multires/halo.rs:31-170, batched: writes
+ * the HSX^3 tile (HSX = B + 2*halo, index z*HSX^2 + y*HSX + x) of every bid in
+ * `bids` (host) to `dev_tiles`.  Faces-only fills write 0 to the x-halo edge
+ * and corner cells a fresh HaloBlock holds there (halo.rs:16-20,154). */
+int msbg_halo_gather(const msbg_grid *g, const int64_t *bids, int64_t n, int halo, int full, int bc,
+                     float *dev_tiles);
+
+/* ---- multires ---------------------------------------------------------------
+ * `fine` is level l (edge 2^L), `coarse` level l+1 (edge 2^(L-1)) over the same
+ * block grid; `dev_levels[bid]` is the block's level (level0,
+ * multires/blockinfo.rs:85). */
+/* fill_multires -- multires/halo.rs:187-358, batched like msbg_halo_gather. */
+int msbg_halo_gather_multires(const msbg_grid *fine, const msbg_grid *coarse, const uint8_t *dev_levels,
+                              const int64_t *bids, int64_t n, int halo, int full, int bc, float *dev_tiles);
+/* The 8-colour sweep of the level-l blocks in `active` with fill_multires
+ * halos (the smoother the multires benches time, multires_bench.rs:143-174). */
+int msbg_sweep_multires(msbg_grid *fine, const msbg_grid *coarse, const uint8_t *dev_levels,
+                        const int64_t *active, int64_t n, int stencil, float dt, int iterations, int clamp01);
+int msbg_sweep_plan_new_multires(msbg_grid *fine, const msbg_grid *coarse, const uint8_t *dev_levels,
+                                 const int64_t *active, int64_t n, msbg_sweep_plan **out);
+/* downsample_channel_avg -- multires/downsample.rs:13-58 for `bids` (host);
+ * the coarse blocks are ensured first. */
+int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t *bids, int64_t n);
+/* regularize_refinement_map -- multires/refinement.rs:66-107 on a device
+ * level map of nx*ny*nz bytes, in place. */
+int msbg_regularize_levels(uint8_t *dev_levels, int64_t nx, int64_t ny, int64_t nz, int n_levels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSBG_HIP_H */

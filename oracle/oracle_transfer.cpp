@@ -75,13 +75,13 @@ void gather_map(const Grid &g, i32 ix0, i32 iy0, i32 iz0, int bc, int dmax, floa
     const float empty = g.empty_value();
     int s = 0;
     for (int k = 0; k < span; ++k) {
-        i32 rz;
+        i32 rz = 0;
         bool okz = resolve_axis(iz0 + k, (i32)g.sz - 1, bc, &rz);
         for (int j = 0; j < span; ++j) {
-            i32 ry;
+            i32 ry = 0;
             bool oky = resolve_axis(iy0 + j, (i32)g.sy - 1, bc, &ry);
             for (int i = 0; i < span; ++i) {
-                i32 rx;
+                i32 rx = 0;
                 bool okx = resolve_axis(ix0 + i, (i32)g.sx - 1, bc, &rx);
                 if (okx && oky && okz) {
                     i64 slot = (((i64)rx >> 4) - bx0) | ((((i64)ry >> 4) - by0) << 1) | ((((i64)rz >> 4) - bz0) << 2);
