@@ -113,6 +113,22 @@ typedef struct msbg_sweep_plan msbg_sweep_plan;
 int msbg_sweep_plan_new(msbg_grid *g, const int64_t *active, int64_t n, msbg_sweep_plan **out);
 int msbg_sweep_plan_free(msbg_sweep_plan *p);
 int msbg_sweep_plan_run_async(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01);
+/* The same passes replayed as one HIP graph (captured on first use for a
+ * given stencil / dt / iterations / clamp01 and cached in the plan). */
+int msbg_sweep_plan_run_graph_async(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01);
+/* The same passes with a HIP event pair around every colour launch: returns
+ * the summed kernel time and the launch count (synchronizes). */
+int msbg_sweep_plan_run_timed(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01,
+                              float *kernel_ms, int64_t *launches);
+
+/* Halo topology -- the neighbour resolution of halo.rs:59-80 (single level,
+ * coarse = NULL) or halo.rs:234-251 (multires): for each bid a row of
+ * (2H+1)^3 int32 codes, index dx + n*dy + n*n*dz, code >= 0 = the neighbour's
+ * slab slot (0 = empty dummy, also outside the domain, 1 = full dummy),
+ * code < 0 = a coarser neighbour at coarse slot -code-1.  host_out holds
+ * n * (2H+1)^3 entries.  A sweep plan stores these rows. */
+int msbg_halo_topology(const msbg_grid *g, const msbg_grid *coarse, const uint8_t *dev_levels, const int64_t *bids,
+                       int64_t n, int halo, int32_t *host_out);
 
 /* HaloBlock::fill::<HALO, FULL> -- multires/halo.rs:31-170, batched: writes
  * the HSX^3 tile (HSX = B + 2*halo, index z*HSX^2 + y*HSX + x) of every bid in

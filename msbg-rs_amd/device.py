@@ -187,6 +187,17 @@ class SweepPlan:
     def run_async(self, stencil, dt, iterations, clamp01=False):
         self.grid.lib.call("msbg_sweep_plan_run_async", self.h, int(stencil), dt, iterations, int(clamp01))
 
+    def run_graph_async(self, stencil, dt, iterations, clamp01=False):
+        """The same passes replayed as one cached HIP graph."""
+        self.grid.lib.call("msbg_sweep_plan_run_graph_async", self.h, int(stencil), dt, iterations, int(clamp01))
+
+    def run_timed(self, stencil, dt, iterations, clamp01=False):
+        """Runs and returns (summed kernel ms, launch count) from per-launch HIP events."""
+        ms, n = C.c_float(), C.c_int64()
+        self.grid.lib.call("msbg_sweep_plan_run_timed", self.h, int(stencil), dt, iterations, int(clamp01),
+                           C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def free(self):
         if getattr(self, "h", None):
             self.grid.lib.call("msbg_sweep_plan_free", self.h)
@@ -206,6 +217,16 @@ def halo_fill(grid: DeviceGrid, bids, halo: int, full: bool, bc=BoundaryConditio
     tiles = grid.lib.empty((b.size, hsx, hsx, hsx), np.float32)
     grid.lib.call("msbg_halo_gather", grid.h, ptr(b), b.size, halo, int(full), int(bc), tiles.ptr)
     return tiles.get()
+
+
+def halo_topology(grid: DeviceGrid, bids, halo: int, coarse: DeviceGrid = None, levels: DeviceArray = None):
+    """Neighbour slot codes per bid, (n, (2H+1)^3) int32 (see msbg_halo_topology)."""
+    b = _i64(bids)
+    nb = (2 * halo + 1) ** 3
+    out = np.zeros((b.size, nb), np.int32)
+    grid.lib.call("msbg_halo_topology", grid.h, coarse.h if coarse else None, levels.ptr if levels else None, ptr(b),
+                  b.size, halo, ptr(out))
+    return out
 
 
 def halo_fill_multires(fine: DeviceGrid, coarse: DeviceGrid, levels: DeviceArray, bids, halo, full,

@@ -50,6 +50,9 @@ _SIGS = {
     "msbg_sweep_plan_new": [vp, vp, i64, PP],
     "msbg_sweep_plan_free": [vp],
     "msbg_sweep_plan_run_async": [vp, i32, f32, i32, i32],
+    "msbg_sweep_plan_run_timed": [vp, i32, f32, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_int64)],
+    "msbg_sweep_plan_run_graph_async": [vp, i32, f32, i32, i32],
+    "msbg_halo_topology": [vp, vp, vp, vp, i64, i32, vp],
     "msbg_halo_gather": [vp, vp, i64, i32, i32, i32, vp],
     "msbg_halo_gather_multires": [vp, vp, vp, vp, i64, i32, i32, i32, vp],
     "msbg_sweep_multires": [vp, vp, vp, vp, i64, i32, f32, i32, i32],

@@ -287,6 +287,34 @@ void orc_halo_fill_multires(i64 sx, i64 sy, i64 sz, int L, const float *fine_sla
 }
 
 // ---------------------------------------------------------------------------
+// Halo topology: the neighbour pointers HaloBlock::fill resolves
+// (halo.rs:59-80) and fill_multires resolves (halo.rs:234-251), written as
+// slab slots.  Entry dx + n*dy + n*n*dz: the neighbour's fine slot (0 = empty
+// dummy, also for blocks outside the block grid), or -(coarse slot) - 1 when
+// levels[b] > levels[bid].  coarse_b2s / levels may be null (single level).
+// ---------------------------------------------------------------------------
+void orc_halo_topology(i64 sx, i64 sy, i64 sz, int L, const i32 *b2s, const i32 *coarse_b2s,
+                       const uint8_t *levels, i64 bid, int halo, i32 *out) {
+    Level g(sx, sy, sz, L, nullptr, b2s);
+    const i32 bx0 = (i32)(bid % g.nx), by0 = (i32)((bid / g.nx) % g.ny), bz0 = (i32)(bid / g.nxy);
+    const int n = 2 * halo + 1;
+    for (int dz = 0; dz < n; ++dz)
+        for (int dy = 0; dy < n; ++dy)
+            for (int dx = 0; dx < n; ++dx) {
+                i32 bx = bx0 + dx - halo, by = by0 + dy - halo, bz = bz0 + dz - halo;
+                i32 code = 0;
+                if (bx >= 0 && bx < g.nx && by >= 0 && by < g.ny && bz >= 0 && bz < g.nz) {
+                    i64 b = (i64)bx + (i64)by * g.nx + (i64)bz * g.nxy;
+                    if (levels && coarse_b2s && levels[b] > levels[bid])
+                        code = -coarse_b2s[b] - 1;
+                    else
+                        code = b2s[b];
+                }
+                out[dx + n * dy + n * n * dz] = code;
+            }
+}
+
+// ---------------------------------------------------------------------------
 // A8: kernel_laplacian -- math/laplacian.rs:16-70 (tile HSX = B+2, out B^3).
 // ---------------------------------------------------------------------------
 void orc_kernel_laplacian(int B, const float *tile, float dt, int clamp01, float *out) {

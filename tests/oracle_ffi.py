@@ -40,6 +40,7 @@ def load():
     _sig(lib, "orc_sweep", None, i64, i64, i64, f32p, i32p, i64p, i64, i32, f32, i32, i32, i32, i32)
     _sig(lib, "orc_sweep_multires", None, i64, i64, i64, i32, f32p, i32p, f32p, i32p, u8p, i64p, i64, i32, f32,
          i32, i32, i32)
+    _sig(lib, "orc_halo_topology", None, i64, i64, i64, i32, i32p, C.c_void_p, C.c_void_p, i64, i32, i32p)
     _sig(lib, "orc_downsample_avg", None, i32, f32p, i32p, f32p, i32p, i64p, i64)
     _sig(lib, "orc_sample_coarse", f32, i64, i64, i64, i32, f32p, i32p, i32, i32, i32)
     _sig(lib, "orc_regularize", None, u8p, i64, i64, i64, i32)

@@ -208,3 +208,51 @@ def test_regularize_matches_oracle(pkg, orc, hip):
     want = levels.copy()
     orc.orc_regularize(want, nx, ny, nz, 4)
     assert np.array_equal(dl.get(), want)
+
+
+@pytest.mark.parametrize("halo", [1, 2])
+def test_halo_topology_matches_oracle(pkg, orc, hip, halo):
+    g = random_grid(pkg, 80, 48, 33, seed=21 + halo, frac=0.5, n_full=3)
+    dev = pkg.DeviceGrid.from_host(g, lib=hip)
+    bids = np.arange(g.n_blocks)
+    got = pkg.halo_topology(dev, bids, halo)
+    nb = (2 * halo + 1) ** 3
+    for i, b in enumerate(bids):
+        want = np.zeros(nb, np.int32)
+        orc.orc_halo_topology(g.sx, g.sy, g.sz, 4, g.b2s, None, None, int(b), halo, want)
+        assert np.array_equal(got[i], want), b
+
+
+def test_halo_topology_multires_matches_oracle(pkg, orc, hip):
+    fine, coarse, levels = multires_pair(pkg, 17)
+    df, dc = pkg.DeviceGrid.from_host(fine, lib=hip), pkg.DeviceGrid.from_host(coarse, lib=hip)
+    dl = hip.to_device(levels)
+    bids = np.arange(fine.n_blocks)
+    for halo in (1, 2):
+        got = pkg.halo_topology(df, bids, halo, coarse=dc, levels=dl)
+        nb = (2 * halo + 1) ** 3
+        for i, b in enumerate(bids):
+            want = np.zeros(nb, np.int32)
+            orc.orc_halo_topology(fine.sx, fine.sy, fine.sz, 4, fine.b2s, coarse.b2s.ctypes.data,
+                                  levels.ctypes.data, int(b), halo, want)
+            assert np.array_equal(got[i], want), (halo, b)
+
+
+@pytest.mark.parametrize("stencil", [0, 1, 2])
+def test_graph_replay_matches_eager(pkg, orc, hip, stencil):
+    """Each graph replay, with eager work and synchronizations between replays,
+    equals the eager launches of the same sweep (bit-exact)."""
+    g = random_grid(pkg, 64, 48, 48, seed=31 + stencil, lo=0.0, hi=1.0)
+    active = np.nonzero(g.b2s >= 2)[0]
+    ga = pkg.DeviceGrid.from_host(g, lib=hip)
+    gb = pkg.DeviceGrid.from_host(g, lib=hip)
+    pa, pb = pkg.SweepPlan(ga, active), pkg.SweepPlan(gb, active)
+    for rep in range(3):
+        pa.run_async(stencil, 0.02, 2, True)
+        hip.synchronize()
+        _ = pkg.halo_fill(gb, active[:3], 1, True)  # eager work between replays
+        pb.run_graph_async(stencil, 0.02, 2, True)
+        hip.synchronize()
+        assert_bitexact(gb.to_host().slab, ga.to_host().slab, "replay %d" % rep)
+    orc.sweep(g, active, stencil, 0.02, 6, True)
+    assert_bitexact(gb.to_host().slab, g.slab, "graph vs oracle")
