@@ -157,6 +157,40 @@ int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t 
  * level map of nx*ny*nz bytes, in place. */
 int msbg_regularize_levels(uint8_t *dev_levels, int64_t nx, int64_t ny, int64_t nz, int n_levels);
 
+/* ---- particles ---------------------------------------------------------------
+ * Particle arrays are device pointers: positions and velocities are n x 3
+ * float (x, y, z interleaved), one particle per row, as fluid/mod.rs:30-36
+ * stores them.  Counts and block ids are below 2^31. */
+
+/* morton3 -- multires/sort.rs:20-22 -- of each particle's voxel (truncated
+ * position, as p2g.rs:153), low 30 bits: exact for voxel coordinates < 1024. */
+int msbg_morton_keys(const float *d_pos, int64_t n, uint32_t *d_keys);
+/* Stable LSD radix sort of (key, value) pairs by the low key_bits bits, in
+ * place (8-bit digits; wave ballot ranking, no global atomics).  The
+ * reference has no radix sort: it is the permutation std::stable_sort gives. */
+int msbg_radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, int64_t n, int key_bits);
+/* Morton sort of particles: d_keys = sorted 30-bit keys, d_perm[i] = original
+ * index of the i-th particle (stable: ties keep input order). */
+int msbg_morton_sort(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm);
+/* Row-major block id of each particle -- p2g.rs:149-157. */
+int msbg_particle_bids(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, int32_t *d_bids);
+/* bucket_indices -- particles/sort.rs:174-272: d_perm (n) = block-major
+ * permutation, stable by block id (the order of the reference's dense oracle
+ * bucket_by_block_dense, particles/sort.rs:455-490; the reference's own
+ * intra-block order is unspecified, :234-237), d_particle_blocks (<= n)
+ * ascending occupied blocks, d_starts (n_occupied + 1) CSR offsets. */
+int msbg_bucket_indices(const int32_t *d_bids, int64_t n, int64_t n_blocks, int32_t *d_perm,
+                        int32_t *d_particle_blocks, int32_t *d_starts, int64_t *n_occupied);
+/* active_blocks -- particles/active.rs:15-47 with footprint_axis
+ * (particles/mod.rs:207-212): union of every particle's footprint blocks
+ * trunc(p/16 -+ r_scan/16) clipped to the block grid, ascending (d_out holds
+ * up to nx*ny*nz entries).  P2G uses r_scan = r_p + 1 (p2g.rs:104-107). */
+int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, float r_scan,
+                       int32_t *d_out, int64_t *n_out);
+/* d_dst[i] = d_src[d_perm[i]] for rows of row_bytes (a multiple of 4): the
+ * payload gather of bucket_particles (fluid/mod.rs:84-89). */
+int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst);
+
 #ifdef __cplusplus
 }
 #endif

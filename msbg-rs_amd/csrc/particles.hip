@@ -1,0 +1,228 @@
+// Particle-side kernels: Morton keys, the Morton sort, block bucketing
+// (bucket_indices) and active-block compaction (active_blocks).
+#include <algorithm>
+
+#include "primitives.hpp"
+
+namespace msbg {
+
+// morton3 (multires/sort.rs:20) of the particle's voxel (trunc position,
+// p2g.rs:153), low 30 bits: exact for voxel coordinates below 1024.
+__global__ void morton_keys_kernel(const float *__restrict__ pos, int64_t n, uint32_t *__restrict__ keys,
+                                   uint32_t *__restrict__ idx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = trunc_index(pos[3 * i]), y = trunc_index(pos[3 * i + 1]), z = trunc_index(pos[3 * i + 2]);
+        keys[i] = spread10(x) | (spread10(y) << 1) | (spread10(z) << 2);
+        if (idx) idx[i] = (uint32_t)i;
+    }
+}
+
+// Row-major block id of a particle (p2g.rs:149-157).
+__global__ void particle_bids_kernel(const float *__restrict__ pos, int64_t n, int32_t nx, int32_t ny,
+                                     uint32_t *__restrict__ bids, uint32_t *__restrict__ idx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = trunc_index(pos[3 * i]), y = trunc_index(pos[3 * i + 1]), z = trunc_index(pos[3 * i + 2]);
+        bids[i] = (x >> 4) + (y >> 4) * (uint32_t)nx + (z >> 4) * (uint32_t)nx * (uint32_t)ny;
+        if (idx) idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void iota_kernel(uint32_t *v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = (uint32_t)i;
+}
+
+__global__ void run_heads_kernel(const uint32_t *__restrict__ keys, int64_t n, uint32_t *__restrict__ flags) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void run_write_kernel(const uint32_t *__restrict__ keys, int64_t n, const uint32_t *__restrict__ heads,
+                                 const uint32_t *__restrict__ rank, int32_t *__restrict__ blocks,
+                                 int32_t *__restrict__ starts) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (heads[i]) {
+            blocks[rank[i]] = (int32_t)keys[i];
+            starts[rank[i]] = (int32_t)i;
+        }
+}
+
+__global__ void set_i32_kernel(int32_t *p, int32_t v) { *p = v; }
+
+// footprint_axis (particles/mod.rs:207-212): trunc(p/bsx -+ r_scan/bsx) clipped.
+__device__ __forceinline__ void footprint(float p, float rb, int32_t nb, int32_t &lo, int32_t &hi) {
+    const float b = p / 16.0f;
+    lo = max((int32_t)truncf(b - rb), 0);
+    hi = min((int32_t)truncf(b + rb), nb - 1);
+}
+
+// active_blocks (particles/active.rs:15-47): every particle marks its footprint
+// blocks (a benign race: all writers store 1).
+__global__ void mark_active_kernel(const float *__restrict__ pos, int64_t n, float rb, int32_t nx, int32_t ny,
+                                   int32_t nz, uint32_t *__restrict__ flags) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x1, x2, y1, y2, z1, z2;
+        footprint(pos[3 * i], rb, nx, x1, x2);
+        footprint(pos[3 * i + 1], rb, ny, y1, y2);
+        footprint(pos[3 * i + 2], rb, nz, z1, z2);
+        for (int32_t bz = z1; bz <= z2; ++bz)
+            for (int32_t by = y1; by <= y2; ++by)
+                for (int32_t bx = x1; bx <= x2; ++bx) flags[bx + (int64_t)by * nx + (int64_t)bz * nx * ny] = 1u;
+    }
+}
+
+__global__ void compact_flags_kernel(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ rank,
+                                     int64_t n, int32_t *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) out[rank[i]] = (int32_t)i;
+}
+
+template <typename T>
+__global__ void gather_kernel_rows(const T *__restrict__ src, const uint32_t *__restrict__ perm, int64_t n, int w,
+                                   T *__restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * w; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / w, c = i - r * w;
+        dst[i] = src[(int64_t)perm[r] * w + c];
+    }
+}
+
+unsigned grid_for(int64_t n, int threads = 256) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + threads - 1) / threads, 65536));
+}
+
+// Occupied blocks and CSR starts of sorted keys; *n_occ is read back.
+int run_lengths(const uint32_t *keys, int64_t n, int32_t *blocks, int32_t *starts, int64_t *n_occ) {
+    int rc;
+    uint32_t *heads = (uint32_t *)workspace(3, sizeof(uint32_t) * 2 * std::max<int64_t>(n, 1), &rc);
+    if (rc) return rc;
+    uint32_t *rank = heads + n;
+    hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), keys, n, heads);
+    MSBG_HIP(hipGetLastError());
+    uint32_t total = 0;
+    rc = scan_exclusive_u32(heads, rank, n, &total);
+    if (rc) return rc;
+    hipLaunchKernelGGL(run_write_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), keys, n, heads, rank, blocks,
+                       starts);
+    hipLaunchKernelGGL(set_i32_kernel, dim3(1), dim3(1), 0, stream(), starts + total, (int32_t)n);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *n_occ = total;
+    return MSBG_OK;
+}
+
+int bits_for(int64_t v) {
+    int b = 1;
+    while (b < 32 && ((int64_t)1 << b) <= v) ++b;
+    return b;
+}
+
+}  // namespace msbg
+
+using namespace msbg;
+
+extern "C" {
+
+int msbg_morton_keys(const float *d_pos, int64_t n, uint32_t *d_keys) {
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_keys)), MSBG_EINVAL, "morton keys: null argument");
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, d_keys, nullptr);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_radix_sort_pairs(uint32_t *d_keys, uint32_t *d_vals, int64_t n, int key_bits) {
+    MSBG_CHECK(n == 0 || (d_keys && d_vals), MSBG_EINVAL, "radix sort: null argument");
+    int rc = radix_sort_pairs(d_keys, d_vals, n, key_bits);
+    if (rc) return rc;
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_morton_sort(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm) {
+    MSBG_CHECK(n >= 0 && n < ((int64_t)1 << 31), MSBG_EINVAL, "morton sort: n must be below 2^31");
+    MSBG_CHECK(n == 0 || (d_pos && d_keys && d_perm), MSBG_EINVAL, "morton sort: null argument");
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, d_keys, d_perm);
+    MSBG_HIP(hipGetLastError());
+    int rc = radix_sort_pairs(d_keys, d_perm, n, 30);
+    if (rc) return rc;
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_particle_bids(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, int32_t *d_bids) {
+    MSBG_CHECK(sx > 0 && sy > 0 && sz > 0 && n >= 0 && (n == 0 || (d_pos && d_bids)), MSBG_EINVAL,
+               "particle bids: bad argument");
+    if (n == 0) return MSBG_OK;
+    const int32_t nx = (int32_t)((sx + 15) >> 4), ny = (int32_t)((sy + 15) >> 4);
+    hipLaunchKernelGGL(particle_bids_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, nx, ny,
+                       (uint32_t *)d_bids, nullptr);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_bucket_indices(const int32_t *d_bids, int64_t n, int64_t n_blocks, int32_t *d_perm,
+                        int32_t *d_particle_blocks, int32_t *d_starts, int64_t *n_occupied) {
+    MSBG_CHECK(n_occupied && d_starts, MSBG_EINVAL, "bucket_indices: null output");
+    MSBG_CHECK(n >= 0 && n < ((int64_t)1 << 31) && n_blocks > 0 && n_blocks < ((int64_t)1 << 31), MSBG_EINVAL,
+               "bucket_indices: n and n_blocks must be below 2^31");
+    MSBG_CHECK(n == 0 || (d_bids && d_perm && d_particle_blocks), MSBG_EINVAL, "bucket_indices: null argument");
+    if (n == 0) {  // particles/sort.rs:176-181
+        hipLaunchKernelGGL(set_i32_kernel, dim3(1), dim3(1), 0, stream(), d_starts, 0);
+        MSBG_HIP(hipGetLastError());
+        MSBG_HIP(hipStreamSynchronize(stream()));
+        *n_occupied = 0;
+        return MSBG_OK;
+    }
+    int rc;
+    uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * n, &rc);
+    if (rc) return rc;
+    MSBG_HIP(hipMemcpyAsync(keys, d_bids, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream()));
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), (uint32_t *)d_perm, n);
+    MSBG_HIP(hipGetLastError());
+    rc = radix_sort_pairs(keys, (uint32_t *)d_perm, n, bits_for(n_blocks - 1));
+    if (rc) return rc;
+    return run_lengths(keys, n, d_particle_blocks, d_starts, n_occupied);
+}
+
+int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, float r_scan,
+                       int32_t *d_out, int64_t *n_out) {
+    MSBG_CHECK(n_out && d_out, MSBG_EINVAL, "active_blocks: null output");
+    MSBG_CHECK(sx > 0 && sy > 0 && sz > 0 && n >= 0 && (n == 0 || d_pos), MSBG_EINVAL, "active_blocks: bad argument");
+    const int32_t nx = (int32_t)((sx + 15) >> 4), ny = (int32_t)((sy + 15) >> 4), nz = (int32_t)((sz + 15) >> 4);
+    const int64_t nb = (int64_t)nx * ny * nz;
+    MSBG_CHECK(nb < ((int64_t)1 << 31), MSBG_EINVAL, "active_blocks: block grid too large for a dense flag array");
+    int rc;
+    uint32_t *flags = (uint32_t *)workspace(5, sizeof(uint32_t) * 2 * nb, &rc);
+    if (rc) return rc;
+    uint32_t *rank = flags + nb;
+    MSBG_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * nb, stream()));
+    if (n > 0)
+        hipLaunchKernelGGL(mark_active_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, r_scan / 16.0f, nx,
+                           ny, nz, flags);
+    MSBG_HIP(hipGetLastError());
+    uint32_t total = 0;
+    rc = scan_exclusive_u32(flags, rank, nb, &total);
+    if (rc) return rc;
+    hipLaunchKernelGGL(compact_flags_kernel, dim3(grid_for(nb)), dim3(256), 0, stream(), flags, rank, nb, d_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *n_out = total;
+    return MSBG_OK;
+}
+
+int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst) {
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_src && d_perm && d_dst)), MSBG_EINVAL, "gather rows: null argument");
+    MSBG_CHECK(row_bytes > 0 && row_bytes % 4 == 0, MSBG_EINVAL, "gather rows: row_bytes must be a multiple of 4");
+    if (n == 0) return MSBG_OK;
+    const int w = (int)(row_bytes / 4);
+    hipLaunchKernelGGL(gather_kernel_rows<uint32_t>, dim3(grid_for(n * w)), dim3(256), 0, stream(),
+                       (const uint32_t *)d_src, d_perm, n, w, (uint32_t *)d_dst);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+}  // extern "C"

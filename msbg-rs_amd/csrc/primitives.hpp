@@ -1,0 +1,35 @@
+// Device-wide primitives shared by the particle kernels: exclusive scan,
+// stable LSD radix sort of (u32 key, u32 value) pairs, and a grow-only
+// scratch workspace.  All run on the library stream.
+#pragma once
+
+#include "internal.hpp"
+
+namespace msbg {
+
+// Grow-only device workspace, one per slot id (not thread safe: one host
+// thread drives the library per device).
+void *workspace(int id, size_t bytes, int *rc);
+
+// out[i] = sum_{j<i} in[i]; returns the total in *total_host if non-null
+// (synchronizes only then).  in and out may alias.
+int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total_host);
+
+// Stable sort of (keys, vals) by the low `key_bits` bits of keys, in place.
+int radix_sort_pairs(uint32_t *keys, uint32_t *vals, int64_t n, int key_bits);
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {  // multires/sort.rs:8-16, 10 bits
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Rust `f32 as usize`: truncation, saturating at 0 for negatives and NaN.
+__device__ __forceinline__ uint32_t trunc_index(float p) {
+    return p > 0.0f ? (p < 4294967040.0f ? (uint32_t)p : 0xffffffffu) : 0u;
+}
+
+}  // namespace msbg

@@ -59,6 +59,13 @@ _SIGS = {
     "msbg_sweep_plan_new_multires": [vp, vp, vp, vp, i64, PP],
     "msbg_downsample_avg": [vp, vp, vp, i64],
     "msbg_regularize_levels": [vp, i64, i64, i64, i32],
+    "msbg_morton_keys": [vp, i64, vp],
+    "msbg_radix_sort_pairs": [vp, vp, i64, i32],
+    "msbg_morton_sort": [vp, i64, vp, vp],
+    "msbg_particle_bids": [i64, i64, i64, vp, i64, vp],
+    "msbg_bucket_indices": [vp, i64, i64, vp, vp, vp, C.POINTER(C.c_int64)],
+    "msbg_active_blocks": [i64, i64, i64, vp, i64, f32, vp, C.POINTER(C.c_int64)],
+    "msbg_gather_rows": [vp, vp, i64, i64, vp],
 }
 _RESTYPE = {"msbg_last_error": C.c_char_p}
 

@@ -1,0 +1,104 @@
+"""Particle-side entry points: Morton sort, bucketing, active blocks.
+
+Mirrors `bucket_indices` / `BucketPlan` (particles/sort.rs:161-272),
+`bucket_particles` (fluid/mod.rs:71-91), `active_blocks` (particles/active.rs:15,
+fluid/p2g.rs:104) and `morton3` (multires/sort.rs:20) for particle voxels.
+Inputs may be numpy arrays (uploaded) or DeviceArrays (used in place).
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .lib import DeviceArray, load
+
+
+def _dev(lib, a, dtype):
+    if isinstance(a, DeviceArray):
+        return a
+    return lib.to_device(np.ascontiguousarray(a, dtype))
+
+
+@dataclass
+class BucketPlan:
+    """particles/sort.rs:161-165 (device arrays; `n_occupied` occupied blocks)."""
+    particle_blocks: DeviceArray
+    starts: DeviceArray
+    perm: DeviceArray
+    n_occupied: int
+
+    def host(self):
+        k = self.n_occupied
+        return self.particle_blocks.get()[:k], self.starts.get()[: k + 1], self.perm.get()
+
+
+def morton_keys(pos, lib=None):
+    lib = lib or load()
+    p = _dev(lib, pos, np.float32)
+    n = p.shape[0] if len(p.shape) > 1 else p.shape[0] // 3
+    keys = lib.empty(n, np.uint32)
+    lib.call("msbg_morton_keys", p.ptr, n, keys.ptr)
+    return keys
+
+
+def radix_sort_pairs(keys, vals, key_bits=32, lib=None):
+    """Stable sort in place of device (keys, vals) by the low key_bits bits."""
+    lib = lib or load()
+    lib.call("msbg_radix_sort_pairs", keys.ptr, vals.ptr, keys.shape[0], key_bits)
+
+
+def morton_sort(pos, lib=None):
+    """(sorted 30-bit keys, permutation) as device arrays."""
+    lib = lib or load()
+    p = _dev(lib, pos, np.float32)
+    n = p.shape[0]
+    keys = lib.empty(n, np.uint32)
+    perm = lib.empty(n, np.uint32)
+    lib.call("msbg_morton_sort", p.ptr, n, keys.ptr, perm.ptr)
+    return keys, perm
+
+
+def particle_bids(sx, sy, sz, pos, lib=None):
+    lib = lib or load()
+    p = _dev(lib, pos, np.float32)
+    n = p.shape[0]
+    bids = lib.empty(n, np.int32)
+    lib.call("msbg_particle_bids", sx, sy, sz, p.ptr, n, bids.ptr)
+    return bids
+
+
+def bucket_indices(bids, n_blocks, lib=None) -> BucketPlan:
+    """particles/sort.rs:174: stable counting order by block id."""
+    lib = lib or load()
+    b = _dev(lib, bids, np.int32)
+    n = b.shape[0]
+    perm = lib.empty(max(n, 1), np.int32)
+    blocks = lib.empty(max(n, 1), np.int32)
+    starts = lib.empty(n + 1, np.int32)
+    k = C.c_int64()
+    lib.call("msbg_bucket_indices", b.ptr, n, n_blocks, perm.ptr, blocks.ptr, starts.ptr, C.byref(k))
+    if n == 0:
+        perm = lib.empty(0, np.int32)
+    return BucketPlan(blocks, starts, perm, k.value)
+
+
+def active_blocks(sx, sy, sz, pos, r_scan, lib=None) -> np.ndarray:
+    """particles/active.rs:15-47: sorted union of footprint blocks (host int64)."""
+    lib = lib or load()
+    p = _dev(lib, pos, np.float32)
+    n = p.shape[0]
+    nb = ((sx + 15) >> 4) * ((sy + 15) >> 4) * ((sz + 15) >> 4)
+    out = lib.empty(nb, np.int32)
+    k = C.c_int64()
+    lib.call("msbg_active_blocks", sx, sy, sz, p.ptr, n, r_scan, out.ptr, C.byref(k))
+    return out.get()[: k.value].astype(np.int64)
+
+
+def gather_rows(src: DeviceArray, perm: DeviceArray, lib=None) -> DeviceArray:
+    """dst[i] = src[perm[i]] row-wise (fluid/mod.rs:84-89)."""
+    lib = lib or load()
+    n = perm.shape[0]
+    row_bytes = src.nbytes // max(src.shape[0], 1)
+    dst = lib.empty((n,) + src.shape[1:], src.dtype)
+    lib.call("msbg_gather_rows", src.ptr, perm.ptr, n, row_bytes, dst.ptr)
+    return dst

@@ -629,24 +629,28 @@ i64 orc_bucket_indices(const i64 *bids, i64 n, i64 n_blocks, i64 *particle_block
     return k;
 }
 
-// Row-major block id of a particle: p2g.rs:149-157 (`as usize` truncation).
+// Rust `f32 as usize` (p2g.rs:153): truncation, saturating at 0 below zero
+// and for NaN (a plain C++ cast of a negative float to unsigned is undefined).
+static inline u64 as_usize(float p) { return p > 0.0f ? (u64)p : 0; }
+
+// Row-major block id of a particle: p2g.rs:149-157.
 void orc_particle_bids(i64 sx, i64 sy, i64 sz, const float *pos, i64 n, i64 *bids) {
     const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4;
     for (i64 i = 0; i < n; ++i) {
-        u64 x = (u64)pos[3 * i], y = (u64)pos[3 * i + 1], z = (u64)pos[3 * i + 2];
+        u64 x = as_usize(pos[3 * i]), y = as_usize(pos[3 * i + 1]), z = as_usize(pos[3 * i + 2]);
         bids[i] = (i64)((x >> 4) + (y >> 4) * nx + (z >> 4) * nx * ny);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Particle Morton keys: morton3 (multires/sort.rs:20) of the particle's voxel
-// (truncated position, p2g.rs:153).  For voxel coordinates < 1024 the code
-// fits the low 30 bits.
+// (truncated position, p2g.rs:153), kept to its low 30 bits -- the full code
+// for voxel coordinates < 1024, which is the 30-bit key the particle sort uses.
 // ---------------------------------------------------------------------------
 void orc_morton_keys(const float *pos, i64 n, u32 *keys) {
     for (i64 i = 0; i < n; ++i) {
-        u64 k = orc_morton3((u64)pos[3 * i], (u64)pos[3 * i + 1], (u64)pos[3 * i + 2]);
-        keys[i] = (u32)k;
+        u64 k = orc_morton3(as_usize(pos[3 * i]), as_usize(pos[3 * i + 1]), as_usize(pos[3 * i + 2]));
+        keys[i] = (u32)(k & 0x3fffffffULL);
     }
 }
 

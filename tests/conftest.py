@@ -22,6 +22,9 @@ def load_pkg():
     return mod
 
 
+load_pkg()  # test modules import `msbg_rs_amd` submodules at collection time
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: long-running CPU test")
