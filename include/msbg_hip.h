@@ -191,6 +191,40 @@ int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, i
  * payload gather of bucket_particles (fluid/mod.rs:84-89). */
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst);
 
+/* ---- grid <-> particle transfers --------------------------------------------- */
+/* Sampler::sample -- math/sample.rs:489-526, with gather_map
+ * (math/gather.rs:54-158): d_out[i] = sample at d_pos[i]; interp 0 = Linear,
+ * 1 = CubicBSpline; align 0 = Corner, 1 = CellCentered; bc as above. */
+int msbg_sample(const msbg_grid *g, const float *d_pos, int64_t n, int interp, int align, int bc, float *d_out);
+/* MacSampler::sample -- fluid/g2p.rs:26-32: face a sampled cell-centred at
+ * p - 1/2 e_a; d_out is n x 3. */
+int msbg_mac_sample(msbg_grid *const *vel, const float *d_pos, int64_t n, int interp, int bc, float *d_out);
+/* g2p_blend -- fluid/g2p.rs:42-45 over n x 3 velocities. */
+int msbg_g2p_blend(const float *d_u_old, const float *d_u, const float *d_u_tilde, int64_t n, float alpha,
+                   float *d_out);
+/* particles_to_grid -- fluid/p2g.rs:141-164: bucket by block, active blocks
+ * (footprint r_p + 1) ensured and zeroed on all six channels
+ * (MacGrid::zero_blocks, mac.rs:95-102), the cubic-falloff splat of mass and
+ * momentum to faces (p2g.rs:169-412) in 8 colour passes per direction with
+ * LDS staging, then u = P/M (mac.rs:106-132).  vel[3] and mass[3] are the face
+ * channels x, y, z over one domain; 0 < r_p <= 3.  The commit adds each margin
+ * region over its own width (the semantics of the reference's naive oracle,
+ * p2g.rs:445-502).  Sums within a staging cell are order-dependent in the last
+ * bits, as in the reference (particles/sort.rs:234-237). */
+int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, const float *d_pos, const float *d_vel,
+                               const float *d_mass, int64_t n, float r_p, int64_t *n_active);
+/* Trilinear (order 1) or quadratic B-spline (order 2) P2G onto a cell-centred
+ * grid (no reference counterpart): blocks under the particles' r_scan = 2
+ * footprint are ensured and zeroed on all four grids; particles are Morton
+ * sorted and every cell gathers its 27 neighbour voxels in a fixed order, so
+ * the result is deterministic.  mass[cell] = sum w m, vel[a][cell] =
+ * sum w m u_a / mass (0 where mass is 0).  d_mass may be NULL (m = 1).
+ * Particles lie in [0, s) per axis, s <= 1024. */
+int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const float *d_pos, const float *d_vel,
+                     const float *d_mass, int64_t n, int64_t *n_active);
+/* The matching G2P: d_out (n x 3) = sum over the support cells of w * vel. */
+int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64_t n, float *d_out);
+
 #ifdef __cplusplus
 }
 #endif

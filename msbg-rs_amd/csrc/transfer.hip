@@ -1,0 +1,636 @@
+// Particle <-> grid transfers.
+//
+//  * Sampler (G2P of the reference): Sample::sample (math/sample.rs:489-526)
+//    with gather_map (math/gather.rs:54-158), per particle, plus the MAC
+//    face sampler (fluid/g2p.rs:26-32) and the FLIP/PIC blend (g2p.rs:42-45).
+//  * MAC P2G of the reference (fluid/p2g.rs:141-412): particles bucketed by
+//    block; per face direction and colour one launch, each workgroup stages
+//    its block's (16+2m)^3 window in LDS (mass + momentum) and commits the 27
+//    regions to the grid; blocks of one colour never overlap, so the commit
+//    needs no global atomics.  Then u = P/M (mac.rs:106-132).
+//  * B-spline P2G / G2P (trilinear, quadratic): particles Morton-sorted by
+//    voxel; every cell gathers its 27 neighbour voxels in a fixed order, so
+//    the result is deterministic and needs neither atomics nor colouring.
+#include <algorithm>
+#include <vector>
+
+#include "primitives.hpp"
+
+namespace msbg {
+
+unsigned grid_for(int64_t n, int threads);
+int run_lengths(const uint32_t *keys, int64_t n, int32_t *blocks, int32_t *starts, int64_t *n_occ);
+
+// ---------------------------------------------------------------------------
+// Reference sampler.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const float *block_ptr(const GridView &g, int64_t bid) {
+    return g.slab + (int64_t)g.b2s[bid] * 4096;
+}
+
+// gather_map (math/gather.rs:54-158).
+__device__ void gather_map_dev(const GridView &g, int ix0, int iy0, int iz0, int bc, int dmax, float *out) {
+    const int span = dmax + 1;
+    if (ix0 >= 0 && iy0 >= 0 && iz0 >= 0 && ix0 + dmax < g.sx && iy0 + dmax < g.sy && iz0 + dmax < g.sz) {
+        const int vx0 = ix0 & 15, vy0 = iy0 & 15, vz0 = iz0 & 15;
+        if (vx0 + dmax < 16 && vy0 + dmax < 16 && vz0 + dmax < 16) {
+            const float *p = block_ptr(g, (ix0 >> 4) + (int64_t)(iy0 >> 4) * g.nx + (int64_t)(iz0 >> 4) * g.nx * g.ny);
+            int s = 0;
+            for (int k = 0; k < span; ++k)
+                for (int j = 0; j < span; ++j)
+                    for (int i = 0; i < span; ++i) out[s++] = p[(vx0 + i) | ((vy0 + j) << 4) | ((vz0 + k) << 8)];
+            return;
+        }
+    }
+    // gather.rs:104-121 reads the resolved voxel through a 2x2x2 table of
+    // blocks anchored at the clamped window corner.  The voxel's own block is
+    // looked up here instead: the same block whenever the table covers it, and
+    // in bounds where it does not (a Neumann reflection across a partial last
+    // block), where the reference's slot bounds check panics (gather.rs:143-146).
+    const float empty = g.slab[0];
+    int s = 0;
+    for (int k = 0; k < span; ++k) {
+        int rz;
+        const bool okz = resolve_axis(iz0 + k, g.sz - 1, bc, rz);
+        for (int j = 0; j < span; ++j) {
+            int ry;
+            const bool oky = resolve_axis(iy0 + j, g.sy - 1, bc, ry);
+            for (int i = 0; i < span; ++i) {
+                int rx;
+                const bool okx = resolve_axis(ix0 + i, g.sx - 1, bc, rx);
+                if (okx && oky && okz) {
+                    const float *p = block_ptr(g, (rx >> 4) + (int64_t)(ry >> 4) * g.nx + (int64_t)(rz >> 4) * g.nx * g.ny);
+                    out[s] = p[(rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8)];
+                } else {
+                    out[s] = empty;
+                }
+                ++s;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float lerp_f(float a, float b, float t) { return __builtin_fmaf(b - a, t, a); }
+
+__device__ __forceinline__ float dot4_seq(const float *f, const float *w) {  // f32x4 reduce_sum, sequential
+    float s = -0.0f;
+    s = s + f[0] * w[0];
+    s = s + f[1] * w[1];
+    s = s + f[2] * w[2];
+    s = s + f[3] * w[3];
+    return s;
+}
+
+__device__ __forceinline__ void cubic_weights_dev(float t, float *w) {  // math/bspline.rs:6-14
+    const float m = 1.0f - t;
+    w[0] = (1.0f / 6.0f) * m * m * m;
+    w[1] = 2.0f / 3.0f - 0.5f * t * t * (2.0f - t);
+    w[2] = 2.0f / 3.0f - 0.5f * m * m * (1.0f + t);
+    w[3] = (1.0f / 6.0f) * t * t * t;
+}
+
+// Sample::sample (math/sample.rs:494-526).  interp 0 Linear, 1 CubicBSpline;
+// align 0 Corner, 1 CellCentered (cubic is always cell-centred).
+__device__ float sample_dev(const GridView &g, float px, float py, float pz, int interp, int align, int bc) {
+    if (interp == 0) {
+        float x = px, y = py, z = pz;
+        if (align == 1) {
+            x = x - 0.5f;
+            y = y - 0.5f;
+            z = z - 0.5f;
+        }
+        const float ix = floorf(x), iy = floorf(y), iz = floorf(z);
+        float c[8];
+        gather_map_dev(g, (int)ix, (int)iy, (int)iz, bc, 1, c);
+        const float u = x - ix, v = y - iy, w = z - iz;
+        const float c00 = lerp_f(c[0], c[1], u), c10 = lerp_f(c[2], c[3], u);
+        const float c01 = lerp_f(c[4], c[5], u), c11 = lerp_f(c[6], c[7], u);
+        return lerp_f(lerp_f(c00, c10, v), lerp_f(c01, c11, v), w);
+    }
+    const float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    float win[64];
+    gather_map_dev(g, (int)xf - 1, (int)yf - 1, (int)zf - 1, bc, 3, win);
+    float wx[4], wy[4], wz[4];
+    cubic_weights_dev(x - xf, wx);
+    cubic_weights_dev(y - yf, wy);
+    cubic_weights_dev(z - zf, wz);
+    float val = 0.0f;
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 4; ++j) val = __builtin_fmaf(dot4_seq(win + 16 * k + 4 * j, wx), wy[j] * wz[k], val);
+    return val;
+}
+
+// out[i * stride] = sample at p_i - 1/2 e_axis (axis < 0: no offset).
+__global__ void sample_kernel(GridView g, const float *__restrict__ pos, int64_t n, int interp, int align, int bc,
+                              int axis, float *__restrict__ out, int stride) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        if (axis >= 0) p[axis] = p[axis] - 0.5f;
+        out[i * stride] = sample_dev(g, p[0], p[1], p[2], interp, align, bc);
+    }
+}
+
+__global__ void blend_kernel(const float *__restrict__ u_old, const float *__restrict__ u,
+                             const float *__restrict__ u_tilde, int64_t n3, float alpha, float *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const float delta = u[i] - u_tilde[i];  // g2p.rs:43-44
+        out[i] = (u_old[i] + delta) * alpha + u[i] * (1.0f - alpha);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reference MAC P2G (fluid/p2g.rs:221-412).
+// ---------------------------------------------------------------------------
+__global__ void bucket_keys_kernel(const float *__restrict__ pos, int64_t n, int32_t nx, int32_t ny,
+                                   uint32_t *__restrict__ keys, uint32_t *__restrict__ idx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = trunc_index(pos[3 * i]), y = trunc_index(pos[3 * i + 1]), z = trunc_index(pos[3 * i + 2]);
+        keys[i] = (x >> 4) + (y >> 4) * (uint32_t)nx + (z >> 4) * (uint32_t)nx * (uint32_t)ny;
+        idx[i] = (uint32_t)i;
+    }
+}
+
+// Gathers (pos, vel, mass) rows into sorted order.
+__global__ void gather_payload_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
+                                      const float *__restrict__ mass, const uint32_t *__restrict__ perm, int64_t n,
+                                      float *__restrict__ spos, float *__restrict__ svel, float *__restrict__ smass) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = perm[i];
+        spos[3 * i] = pos[3 * j];
+        spos[3 * i + 1] = pos[3 * j + 1];
+        spos[3 * i + 2] = pos[3 * j + 2];
+        svel[3 * i] = vel[3 * j];
+        svel[3 * i + 1] = vel[3 * j + 1];
+        svel[3 * i + 2] = vel[3 * j + 2];
+        if (smass) smass[i] = mass ? mass[j] : 1.0f;
+    }
+}
+
+// One (direction a, colour) pass: workgroup k stages the particles of
+// occupied block particle_blocks[list[k]] and commits its window.
+__global__ __launch_bounds__(256) void mac_splat_kernel(GridView vg, GridView mg, int a, const int32_t *__restrict__ list,
+                                                        const int32_t *__restrict__ pblocks,
+                                                        const int32_t *__restrict__ starts,
+                                                        const float *__restrict__ spos, const float *__restrict__ svel,
+                                                        const float *__restrict__ smass, float r_p, float inv_r2, int m) {
+    extern __shared__ float smem[];
+    const int MSX = 16 + 2 * m, NS = MSX * MSX * MSX;
+    float *sm = smem, *sv = smem + NS;
+    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
+        sm[s] = 0.0f;
+        sv[s] = 0.0f;
+    }
+    __syncthreads();
+    const int k = list[blockIdx.x];
+    const int bid = pblocks[k];
+    const int bx = bid % vg.nx, by = (bid / vg.nx) % vg.ny, bz = bid / (vg.nx * vg.ny);
+    const int b0x = bx * 16, b0y = by * 16, b0z = bz * 16;
+    const int x0s = b0x - m, y0s = b0y - m, z0s = b0z - m, whi = 16 + m - 1;
+    const float ofx = a == 0 ? 1.0f : 0.5f, ofy = a == 1 ? 1.0f : 0.5f, ofz = a == 2 ? 1.0f : 0.5f;
+    for (int pi = starts[k] + threadIdx.x; pi < starts[k + 1]; pi += blockDim.x) {  // p2g.rs:261-333
+        const float px = spos[3 * pi], py = spos[3 * pi + 1], pz = spos[3 * pi + 2];
+        const float scale_mass = smass[pi];
+        const float scale_mom = smass[pi] * svel[3 * pi + a];
+        const float px0 = px - 0.5f, py0 = py - 0.5f, pz0 = pz - 0.5f;
+        const int ix1 = min(max((int)ceilf(px - r_p - ofx), b0x - m), b0x + whi);
+        const int ix2 = min(max((int)floorf(px + r_p - ofx), b0x - m), b0x + whi);
+        const int iy1 = min(max((int)ceilf(py - r_p - ofy), b0y - m), b0y + whi);
+        const int iy2 = min(max((int)floorf(py + r_p - ofy), b0y - m), b0y + whi);
+        const int iz1 = min(max((int)ceilf(pz - r_p - ofz), b0z - m), b0z + whi);
+        const int iz2 = min(max((int)floorf(pz + r_p - ofz), b0z - m), b0z + whi);
+        for (int iz = iz1; iz <= iz2; ++iz) {
+            const float dz = (float)iz - pz0;
+            const float dz2 = dz * dz;
+            for (int iy = iy1; iy <= iy2; ++iy) {
+                const float dy = (float)iy - py0;
+                const float dyz = dz2 + dy * dy;
+                const int row = ((iz - z0s) * MSX + (iy - y0s)) * MSX;
+                for (int ix = ix1; ix <= ix2; ++ix) {
+                    const float dx = (float)ix - px0;
+                    const float dist_sq = dx * dx + dyz;
+                    const float d_a = a == 0 ? dx : (a == 1 ? dy : dz);
+                    const float df2 = dist_sq + d_a + 0.25f;
+                    float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
+                    w = w * w * w;
+                    atomicAdd(&sm[row + ix - x0s], w * scale_mass);
+                    atomicAdd(&sv[row + ix - x0s], w * scale_mom);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Commit (p2g.rs:342-411): staging cell -> its region -> the region's block.
+    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
+        const float vm = sm[s], vv = sv[s];
+        if (vm == 0.0f && vv == 0.0f) continue;
+        const int lx = s % MSX, ly = (s / MSX) % MSX, lz = s / (MSX * MSX);
+        const int dx = lx < m ? -1 : (lx < m + 16 ? 0 : 1);
+        const int dy = ly < m ? -1 : (ly < m + 16 ? 0 : 1);
+        const int dz = lz < m ? -1 : (lz < m + 16 ? 0 : 1);
+        const int xlo = dx < 0 ? 0 : (dx == 0 ? m : m + 16), xlen = dx == 0 ? 16 : m;
+        const int ylo = dy < 0 ? 0 : (dy == 0 ? m : m + 16), ylen = dy == 0 ? 16 : m;
+        const int zlo = dz < 0 ? 0 : (dz == 0 ? m : m + 16), zlen = dz == 0 ? 16 : m;
+        const int gx0 = b0x + xlo - m, gy0 = b0y + ylo - m, gz0 = b0z + zlo - m;
+        if (gx0 < 0 || gy0 < 0 || gz0 < 0) continue;
+        if (gx0 + xlen > vg.sx || gy0 + ylen > vg.sy || gz0 + zlen > vg.sz) continue;
+        const int64_t nb = (int64_t)(bx + dx) + (int64_t)(by + dy) * vg.nx + (int64_t)(bz + dz) * vg.nx * vg.ny;
+        const int gx = x0s + lx, gy = y0s + ly, gz = z0s + lz;
+        const int vid = (gx & 15) | ((gy & 15) << 4) | ((gz & 15) << 8);
+        const int ms = mg.b2s[nb], vs = vg.b2s[nb];
+        if (ms < 2 || vs < 2) continue;
+        if (vm != 0.0f) mg.slab[(int64_t)ms * 4096 + vid] += vm;
+        if (vv != 0.0f) vg.slab[(int64_t)vs * 4096 + vid] += vv;
+    }
+}
+
+// MacGrid::normalize_velocity (mac.rs:106-132): u = P/M, 0 where M <= 0.
+__global__ void normalize_kernel(GridView vg, GridView mg, const int32_t *__restrict__ active) {
+    const int64_t bid = active[blockIdx.x];
+    const int vs = vg.b2s[bid], ms = mg.b2s[bid];
+    if (vs < 2) return;
+    for (int j = threadIdx.x; j < 4096; j += blockDim.x) {
+        const float mm = mg.slab[(int64_t)ms * 4096 + j];
+        float &v = vg.slab[(int64_t)vs * 4096 + j];
+        v = mm > 0.0f ? v / mm : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B-spline transfers.
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__device__ __forceinline__ float bspline_w(float p, int ci) {
+    const float x = (p - 0.5f) - (float)ci;
+    const float ax = fabsf(x);
+    if (ORDER == 1) return ax < 1.0f ? 1.0f - ax : 0.0f;
+    if (ax < 0.5f) return 0.75f - x * x;
+    if (ax < 1.5f) {
+        const float t = 1.5f - ax;
+        return 0.5f * t * t;
+    }
+    return 0.0f;
+}
+
+// Voxel ranges of the sorted particles: vbeg/vend indexed by slot*4096+vid.
+__global__ void voxel_ranges_kernel(const uint32_t *__restrict__ keys, const float *__restrict__ spos, int64_t n,
+                                    GridView g, uint32_t *__restrict__ vbeg, uint32_t *__restrict__ vend) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool head = i == 0 || keys[i - 1] != keys[i];
+        const bool tail = i == n - 1 || keys[i + 1] != keys[i];
+        if (!head && !tail) continue;
+        const uint32_t x = trunc_index(spos[3 * i]), y = trunc_index(spos[3 * i + 1]), z = trunc_index(spos[3 * i + 2]);
+        if (x >= (uint32_t)g.sx || y >= (uint32_t)g.sy || z >= (uint32_t)g.sz) continue;
+        const int slot = g.b2s[(x >> 4) + (int64_t)(y >> 4) * g.nx + (int64_t)(z >> 4) * g.nx * g.ny];
+        if (slot < 2) continue;
+        const int64_t c = (int64_t)slot * 4096 + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8));
+        if (head) vbeg[c] = (uint32_t)i;
+        if (tail) vend[c] = (uint32_t)(i + 1);
+    }
+}
+
+// Workgroup per active block, 256 lanes x 16 cells (z walk).  Each cell sums
+// its 27 neighbour voxels (z, y, x from -1 to 1) and each voxel's particles in
+// sorted order, exactly as the oracle does.
+template <int ORDER>
+__global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView v0, GridView v1, GridView v2,
+                                                          const int32_t *__restrict__ active,
+                                                          const uint32_t *__restrict__ vbeg,
+                                                          const uint32_t *__restrict__ vend,
+                                                          const float *__restrict__ spos,
+                                                          const float *__restrict__ svel,
+                                                          const float *__restrict__ smass) {
+    __shared__ int s_nb[27];
+    const int64_t bid = active[blockIdx.x];
+    const int bx = (int)(bid % mg.nx), by = (int)((bid / mg.nx) % mg.ny), bz = (int)(bid / ((int64_t)mg.nx * mg.ny));
+    if (threadIdx.x < 27) {
+        const int t = threadIdx.x;
+        const int nx = bx + t % 3 - 1, ny = by + (t / 3) % 3 - 1, nz = bz + t / 9 - 1;
+        s_nb[t] = (nx >= 0 && nx < mg.nx && ny >= 0 && ny < mg.ny && nz >= 0 && nz < mg.nz)
+                      ? mg.b2s[nx + (int64_t)ny * mg.nx + (int64_t)nz * mg.nx * mg.ny]
+                      : 0;
+    }
+    __syncthreads();
+    const int ms = s_nb[13];
+    const int s0 = v0.b2s[bid], s1 = v1.b2s[bid], s2 = v2.b2s[bid];
+    if (ms < 2) return;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int cx = bx * 16 + lx, cy = by * 16 + ly;
+    for (int lz = 0; lz < 16; ++lz) {
+        const int cz = bz * 16 + lz;
+        float am = 0.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int vz = cz + dz;
+            if (vz < 0 || vz >= mg.sz) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int vy = cy + dy;
+                if (vy < 0 || vy >= mg.sy) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int vx = cx + dx;
+                    if (vx < 0 || vx >= mg.sx) continue;
+                    const int nbi = ((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1);
+                    const int slot = s_nb[nbi];
+                    if (slot < 2) continue;
+                    const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
+                    const uint32_t e = vend[c];
+                    for (uint32_t j = vbeg[c]; j < e; ++j) {
+                        const float w = (bspline_w<ORDER>(spos[3 * j], cx) * bspline_w<ORDER>(spos[3 * j + 1], cy)) *
+                                        bspline_w<ORDER>(spos[3 * j + 2], cz);
+                        const float wm = w * smass[j];
+                        am = am + wm;
+                        a0 = a0 + wm * svel[3 * j];
+                        a1 = a1 + wm * svel[3 * j + 1];
+                        a2 = a2 + wm * svel[3 * j + 2];
+                    }
+                }
+            }
+        }
+        const int vid = lx | (ly << 4) | (lz << 8);
+        mg.slab[(int64_t)ms * 4096 + vid] = am;
+        const bool pos_m = am > 0.0f;
+        v0.slab[(int64_t)s0 * 4096 + vid] = pos_m ? a0 / am : 0.0f;
+        v1.slab[(int64_t)s1 * 4096 + vid] = pos_m ? a1 / am : 0.0f;
+        v2.slab[(int64_t)s2 * 4096 + vid] = pos_m ? a2 / am : 0.0f;
+    }
+}
+
+template <int ORDER>
+__global__ void g2p_bspline_kernel(GridView v0, GridView v1, GridView v2, const float *__restrict__ pos, int64_t n,
+                                   float *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        const int fx = (int)floorf(px), fy = (int)floorf(py), fz = (int)floorf(pz);
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int cz = fz + dz;
+            if (cz < 0 || cz >= v0.sz) continue;
+            const float wz = bspline_w<ORDER>(pz, cz);
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int cy = fy + dy;
+                if (cy < 0 || cy >= v0.sy) continue;
+                const float wy = bspline_w<ORDER>(py, cy);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int cx = fx + dx;
+                    if (cx < 0 || cx >= v0.sx) continue;
+                    const float w = (bspline_w<ORDER>(px, cx) * wy) * wz;
+                    const int64_t bid = (cx >> 4) + (int64_t)(cy >> 4) * v0.nx + (int64_t)(cz >> 4) * v0.nx * v0.ny;
+                    const int vid = (cx & 15) | ((cy & 15) << 4) | ((cz & 15) << 8);
+                    a0 = a0 + w * v0.slab[(int64_t)v0.b2s[bid] * 4096 + vid];
+                    a1 = a1 + w * v1.slab[(int64_t)v1.b2s[bid] * 4096 + vid];
+                    a2 = a2 + w * v2.slab[(int64_t)v2.b2s[bid] * 4096 + vid];
+                }
+            }
+        }
+        out[3 * i] = a0;
+        out[3 * i + 1] = a1;
+        out[3 * i + 2] = a2;
+    }
+}
+
+// Ensures `bids` (host) on every grid and zero-fills them (MacGrid::zero_blocks,
+// mac.rs:95-102).
+int zero_blocks(msbg_grid *const *grids, int ng, const std::vector<int64_t> &bids) {
+    for (int k = 0; k < ng; ++k) {
+        int rc = msbg_grid_ensure_blocks(grids[k], bids.data(), (int64_t)bids.size());
+        if (rc) return rc;
+        rc = msbg_grid_fill_blocks(grids[k], bids.data(), (int64_t)bids.size(), 0.0f);
+        if (rc) return rc;
+    }
+    return MSBG_OK;
+}
+
+int check_same_dims(msbg_grid *const *grids, int ng) {
+    for (int k = 0; k < ng; ++k) {
+        MSBG_CHECK(grids[k], MSBG_EINVAL, "transfer: null grid");
+        MSBG_CHECK(grids[k]->L == 4 && grids[k]->sx == grids[0]->sx && grids[k]->sy == grids[0]->sy &&
+                       grids[k]->sz == grids[0]->sz,
+                   MSBG_EINVAL, "transfer: all channels must be 16^3-block grids over one domain");
+    }
+    return MSBG_OK;
+}
+
+// Active blocks of the particles (device list), downloaded.
+int active_list(const msbg_grid *g, const float *d_pos, int64_t n, float r_scan, std::vector<int64_t> &out,
+                int32_t **d_list) {
+    int rc;
+    int32_t *d_act = (int32_t *)workspace(6, sizeof(int32_t) * std::max<int64_t>(g->n_blocks, 1), &rc);
+    if (rc) return rc;
+    int64_t k = 0;
+    rc = msbg_active_blocks(g->sx, g->sy, g->sz, d_pos, n, r_scan, d_act, &k);
+    if (rc) return rc;
+    std::vector<int32_t> h((size_t)k);
+    if (k) MSBG_HIP(hipMemcpy(h.data(), d_act, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
+    out.assign(h.begin(), h.end());
+    *d_list = d_act;
+    return MSBG_OK;
+}
+
+}  // namespace msbg
+
+using namespace msbg;
+
+extern "C" {
+
+int msbg_sample(const msbg_grid *g, const float *d_pos, int64_t n, int interp, int align, int bc, float *d_out) {
+    MSBG_CHECK(g && g->L == 4, MSBG_EINVAL, "sample: 16^3-block grid required");
+    MSBG_CHECK(interp == 0 || interp == 1, MSBG_EINVAL, "sample: interp is 0 (Linear) or 1 (CubicBSpline)");
+    MSBG_CHECK((align == 0 || align == 1) && bc >= 0 && bc <= 2, MSBG_EINVAL, "sample: bad alignment / boundary");
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "sample: null argument");
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(sample_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), g->view(), d_pos, n, interp, align,
+                       bc, -1, d_out, 1);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_mac_sample(msbg_grid *const *vel, const float *d_pos, int64_t n, int interp, int bc, float *d_out) {
+    MSBG_CHECK(vel, MSBG_EINVAL, "mac sample: null grids");
+    int rc = check_same_dims(vel, 3);
+    if (rc) return rc;
+    MSBG_CHECK(interp == 0 || interp == 1, MSBG_EINVAL, "mac sample: interp is 0 or 1");
+    MSBG_CHECK(bc >= 0 && bc <= 2, MSBG_EINVAL, "mac sample: bad boundary");
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "mac sample: null argument");
+    if (n == 0) return MSBG_OK;
+    for (int a = 0; a < 3; ++a)
+        hipLaunchKernelGGL(sample_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[a]->view(), d_pos, n,
+                           interp, 1, bc, a, d_out + a, 3);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_g2p_blend(const float *d_u_old, const float *d_u, const float *d_u_tilde, int64_t n, float alpha,
+                   float *d_out) {
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_u_old && d_u && d_u_tilde && d_out)), MSBG_EINVAL, "g2p blend: null argument");
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(blend_kernel, dim3(grid_for(3 * n, 256)), dim3(256), 0, stream(), d_u_old, d_u, d_u_tilde, 3 * n,
+                       alpha, d_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, const float *d_pos, const float *d_vel,
+                               const float *d_mass, int64_t n, float r_p, int64_t *n_active) {
+    MSBG_CHECK(vel && mass && n_active, MSBG_EINVAL, "particles_to_grid: null argument");
+    msbg_grid *all[6] = {vel[0], vel[1], vel[2], mass[0], mass[1], mass[2]};
+    int rc = check_same_dims(all, 6);
+    if (rc) return rc;
+    // splat precondition (p2g.rs:32-42, 174-181): 0 < r_p <= margin, staging fits LDS.
+    MSBG_CHECK(r_p > 0.0f && r_p <= 3.0f, MSBG_EINVAL, "particles_to_grid: r_p must be in (0, 3]");
+    MSBG_CHECK(n >= 0 && n < ((int64_t)1 << 31) && (n == 0 || (d_pos && d_vel && d_mass)), MSBG_EINVAL,
+               "particles_to_grid: bad particle arrays");
+    const msbg_grid *g = vel[0];
+    // active blocks with the (r_p + 1)/bsx footprint (p2g.rs:104-107), zeroed
+    std::vector<int64_t> act;
+    int32_t *d_act = nullptr;
+    rc = active_list(g, d_pos, n, r_p + 1.0f, act, &d_act);
+    if (rc) return rc;
+    rc = zero_blocks(all, 6, act);
+    if (rc) return rc;
+    *n_active = (int64_t)act.size();
+    if (n == 0) return MSBG_OK;
+    // bucket by block (p2g.rs:149-158)
+    uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n, &rc);
+    if (rc) return rc;
+    uint32_t *perm = keys + n;
+    hipLaunchKernelGGL(bucket_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, n, (int32_t)g->nx,
+                       (int32_t)g->ny, keys, perm);
+    MSBG_HIP(hipGetLastError());
+    int key_bits = 1;
+    while (key_bits < 32 && ((int64_t)1 << key_bits) < g->n_blocks) ++key_bits;
+    rc = radix_sort_pairs(keys, perm, n, key_bits);
+    if (rc) return rc;
+    float *pay = (float *)workspace(7, sizeof(float) * 7 * n + sizeof(int32_t) * (2 * n + 2), &rc);
+    if (rc) return rc;
+    float *spos = pay, *svel = pay + 3 * n, *smass = pay + 6 * n;
+    int32_t *pblocks = (int32_t *)(pay + 7 * n), *starts = pblocks + n;
+    hipLaunchKernelGGL(gather_payload_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass, perm,
+                       n, spos, svel, smass);
+    MSBG_HIP(hipGetLastError());
+    int64_t nocc = 0;
+    rc = run_lengths(keys, n, pblocks, starts, &nocc);
+    if (rc) return rc;
+    // colour lists of occupied-block indices (p2g.rs:186-191)
+    std::vector<int32_t> hb((size_t)nocc);
+    MSBG_HIP(hipMemcpy(hb.data(), pblocks, sizeof(int32_t) * nocc, hipMemcpyDeviceToHost));
+    std::vector<int32_t> order;
+    int64_t cnt[8] = {0};
+    for (int c = 0; c < 8; ++c)
+        for (int64_t i = 0; i < nocc; ++i) {
+            const int64_t b = hb[i];
+            const int64_t bx = b % g->nx, by = (b / g->nx) % g->ny, bz = b / g->nxy;
+            if ((int)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) == c) {
+                order.push_back((int32_t)i);
+                ++cnt[c];
+            }
+        }
+    int32_t *d_order = (int32_t *)workspace(3, sizeof(int32_t) * std::max<int64_t>(nocc, 1), &rc);
+    if (rc) return rc;
+    MSBG_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * nocc, hipMemcpyHostToDevice, stream()));
+    const int m = (int)ceilf(r_p);
+    const int msx = 16 + 2 * m;
+    const size_t lds = sizeof(float) * 2 * msx * msx * msx;
+    {
+        int dev = 0, max_lds = 0;
+        MSBG_HIP(hipGetDevice(&dev));
+        MSBG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        MSBG_CHECK((int64_t)lds <= max_lds, MSBG_EINVAL,
+                   "particles_to_grid: the staging window for this r_p exceeds the workgroup LDS limit");
+        if (lds > 65536)
+            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds));
+    }
+    const float inv_r2 = 1.0f / (r_p * r_p);
+    for (int a = 0; a < 3; ++a) {
+        int64_t off = 0;
+        for (int c = 0; c < 8; ++c) {
+            if (cnt[c])
+                hipLaunchKernelGGL(mac_splat_kernel, dim3((unsigned)cnt[c]), dim3(256), lds, stream(), vel[a]->view(),
+                                   mass[a]->view(), a, d_order + off, pblocks, starts, spos, svel, smass, r_p, inv_r2, m);
+            off += cnt[c];
+        }
+        MSBG_HIP(hipGetLastError());
+    }
+    if (!act.empty())
+        for (int a = 0; a < 3; ++a)
+            hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)act.size()), dim3(256), 0, stream(), vel[a]->view(),
+                               mass[a]->view(), d_act);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const float *d_pos, const float *d_vel,
+                     const float *d_mass, int64_t n, int64_t *n_active) {
+    MSBG_CHECK(order == 1 || order == 2, MSBG_EINVAL, "p2g_bspline: order is 1 (trilinear) or 2 (quadratic)");
+    MSBG_CHECK(mass && vel && n_active, MSBG_EINVAL, "p2g_bspline: null argument");
+    msbg_grid *all[4] = {mass, vel[0], vel[1], vel[2]};
+    int rc = check_same_dims(all, 4);
+    if (rc) return rc;
+    MSBG_CHECK(n >= 0 && n < ((int64_t)1 << 31) && (n == 0 || (d_pos && d_vel)), MSBG_EINVAL,
+               "p2g_bspline: bad particle arrays");
+    MSBG_CHECK(mass->sx <= 1024 && mass->sy <= 1024 && mass->sz <= 1024, MSBG_EINVAL,
+               "p2g_bspline: the 30-bit Morton key needs a domain of at most 1024^3");
+    // support cells floor(p)-1 .. floor(p)+1 lie in the footprint blocks of r_scan 2
+    std::vector<int64_t> act;
+    int32_t *d_act = nullptr;
+    rc = active_list(mass, d_pos, n, 2.0f, act, &d_act);
+    if (rc) return rc;
+    rc = zero_blocks(all, 4, act);
+    if (rc) return rc;
+    *n_active = (int64_t)act.size();
+    if (n == 0 || act.empty()) return MSBG_OK;
+    // Morton sort (stable) and the sorted payload
+    uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n, &rc);
+    if (rc) return rc;
+    uint32_t *perm = keys + n;
+    rc = msbg_morton_sort(d_pos, n, keys, perm);
+    if (rc) return rc;
+    float *pay = (float *)workspace(7, sizeof(float) * 7 * n, &rc);
+    if (rc) return rc;
+    float *spos = pay, *svel = pay + 3 * n, *smass = pay + 6 * n;
+    hipLaunchKernelGGL(gather_payload_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass, perm,
+                       n, spos, svel, smass);
+    MSBG_HIP(hipGetLastError());
+    // per-voxel ranges of the sorted particles, indexed like the mass slab
+    const int64_t ncell = mass->n_slots * 4096;
+    uint32_t *vr = (uint32_t *)workspace(8, sizeof(uint32_t) * 2 * ncell, &rc);
+    if (rc) return rc;
+    uint32_t *vbeg = vr, *vend = vr + ncell;
+    MSBG_HIP(hipMemsetAsync(vr, 0, sizeof(uint32_t) * 2 * ncell, stream()));
+    hipLaunchKernelGGL(voxel_ranges_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), keys, spos, n, mass->view(),
+                       vbeg, vend);
+    MSBG_HIP(hipGetLastError());
+    const dim3 grid((unsigned)act.size()), block(256);
+    if (order == 1)
+        hipLaunchKernelGGL(p2g_bspline_kernel<1>, grid, block, 0, stream(), mass->view(), vel[0]->view(), vel[1]->view(),
+                           vel[2]->view(), d_act, vbeg, vend, spos, svel, smass);
+    else
+        hipLaunchKernelGGL(p2g_bspline_kernel<2>, grid, block, 0, stream(), mass->view(), vel[0]->view(), vel[1]->view(),
+                           vel[2]->view(), d_act, vbeg, vend, spos, svel, smass);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64_t n, float *d_out) {
+    MSBG_CHECK(order == 1 || order == 2, MSBG_EINVAL, "g2p_bspline: order is 1 (trilinear) or 2 (quadratic)");
+    MSBG_CHECK(vel, MSBG_EINVAL, "g2p_bspline: null grids");
+    int rc = check_same_dims(vel, 3);
+    if (rc) return rc;
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "g2p_bspline: null argument");
+    if (n == 0) return MSBG_OK;
+    if (order == 1)
+        hipLaunchKernelGGL(g2p_bspline_kernel<1>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
+                           vel[1]->view(), vel[2]->view(), d_pos, n, d_out);
+    else
+        hipLaunchKernelGGL(g2p_bspline_kernel<2>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
+                           vel[1]->view(), vel[2]->view(), d_pos, n, d_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+}  // extern "C"

@@ -72,6 +72,15 @@ void gather_map(const Grid &g, i32 ix0, i32 iy0, i32 iz0, int bc, int dmax, floa
                     bzz = std::min(bz0 + sz, g.nz - 1);
                 slots[sx | (sy << 1) | (sz << 2)] = g.block_ptr(bxx + byy * g.nx + bzz * g.nxy);
             }
+    // Where a Neumann reflection across a partial last block resolves outside
+    // the table (slot index outside 0..7), the reference's bounds check panics
+    // (gather.rs:143-146); the voxel's own block is read instead, which is the
+    // table entry whenever the table covers it.
+    auto slot_ptr = [&](i32 rx, i32 ry, i32 rz) -> const float * {
+        const i64 ox = ((i64)rx >> 4) - bx0, oy = ((i64)ry >> 4) - by0, oz = ((i64)rz >> 4) - bz0;
+        if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1 && oz >= 0 && oz <= 1) return slots[ox | (oy << 1) | (oz << 2)];
+        return g.block_ptr(((i64)rx >> 4) + ((i64)ry >> 4) * g.nx + ((i64)rz >> 4) * g.nxy);
+    };
     const float empty = g.empty_value();
     int s = 0;
     for (int k = 0; k < span; ++k) {
@@ -84,8 +93,7 @@ void gather_map(const Grid &g, i32 ix0, i32 iy0, i32 iz0, int bc, int dmax, floa
                 i32 rx = 0;
                 bool okx = resolve_axis(ix0 + i, (i32)g.sx - 1, bc, &rx);
                 if (okx && oky && okz) {
-                    i64 slot = (((i64)rx >> 4) - bx0) | ((((i64)ry >> 4) - by0) << 1) | ((((i64)rz >> 4) - bz0) << 2);
-                    out[s] = slots[slot][(rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8)];
+                    out[s] = slot_ptr(rx, ry, rz)[(rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8)];
                 } else {
                     out[s] = empty;
                 }

@@ -56,6 +56,9 @@ def load():
     _sig(lib, "orc_p2g_mac", None, i64, i64, i64, f32p, f32p, f32p, i64p, i64p, i64, f32, f32p, i64, i32p,
          i64p, i64, i32)
     _sig(lib, "orc_p2g_mac_naive", None, i64, i64, i64, f32p, f32p, f32p, i64, f32, f32p)
+    _sig(lib, "orc_bspline_weight", f32, i32, f32, i64)
+    _sig(lib, "orc_p2g_bspline", None, i32, i64, i64, i64, f32p, f32p, C.c_void_p, i64, i32p, i64, i64p, i64, f32p)
+    _sig(lib, "orc_g2p_bspline", None, i32, i64, i64, i64, f32p, i64, i32p, f32p, i64, f32p)
     _LIB = Oracle(lib)
     return _LIB
 

@@ -1,0 +1,129 @@
+"""GPU parity of the transfers: the reference sampler, MAC P2G and the
+B-spline P2G / G2P against the oracle."""
+import numpy as np
+import pytest
+
+import test_oracle_goldens as G
+from msbg_rs_amd import transfer as T
+from test_gpu_sweep import assert_bitexact, random_grid
+from transfer_helpers import dense_from_slabs, oracle_g2p_bspline, oracle_p2g_bspline, oracle_p2g_mac
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("interp", [0, 1])
+@pytest.mark.parametrize("align", [0, 1])
+@pytest.mark.parametrize("bc", [0, 1, 2])
+def test_sample_matches_oracle(pkg, orc, hip, interp, align, bc):
+    g = random_grid(pkg, 40, 33, 48, seed=40 + interp + 2 * align + 4 * bc, frac=0.8)
+    dev = pkg.DeviceGrid.from_host(g, lib=hip)
+    rng = np.random.default_rng(bc)
+    pos = rng.uniform(-3.0, [43.0, 36.0, 51.0], size=(20000, 3)).astype(np.float32)
+    pos[:4] = [[0, 0, 0], [15.5, 15.5, 15.5], [16, 16, 16], [39.9, 32.9, 47.9]]
+    got = T.sample(dev, pos, interp, align, bc).get()
+    want, _, _ = orc.sample(g, pos, interp, align, bc)
+    assert_bitexact(got, want, "sample")
+
+
+def test_sample_interp_golden_on_device(pkg, hip):
+    d = G.GOLD["difftest_interp"]
+    g = pkg.SparseGrid("interp", 48, 48, 48, 0.0, 1.0)
+    g.fill_from_function(np.arange(g.n_blocks), G.interp_field)
+    dev = pkg.DeviceGrid.from_host(g, lib=hip)
+    pos = np.array(d["positions"], np.float32)
+    gold = np.array(d["golden"], np.float32)
+    lin = T.sample(dev, pos, 0, 0, 0).get()
+    cub = T.sample(dev, pos, 1, 0, 0).get()
+    np.testing.assert_allclose(lin, gold[:, 0], atol=1e-5)
+    np.testing.assert_allclose(cub, gold[:, 4], atol=1e-5)
+
+
+def mac_from_slabs(pkg, hip, b2s, slabs, s):
+    mac = T.MacGrid(*s, capacity_blocks=slabs.shape[1], lib=hip)
+    for a in range(3):
+        host = pkg.SparseGrid("v", *s, 0.0, 0.0)
+        host.b2s = b2s.astype(np.int32)
+        host._slab = np.ascontiguousarray(slabs[a])
+        host.n_slots = slabs.shape[1]
+        hip.call("msbg_grid_load", mac.velocity[a].h, host.b2s.ctypes.data, host._slab.ctypes.data, host.n_slots)
+    return mac
+
+
+def test_mac_sampler_and_blend(pkg, orc, hip):
+    from test_oracle_transfer import face_field_slabs
+    b2s, slabs = face_field_slabs()
+    mac = mac_from_slabs(pkg, hip, b2s, slabs, (32, 32, 32))
+    got = T.MacSampler(mac, pkg.BoundaryCondition.Clamp).sample(np.array([[5.0, 8.5, 8.5]], np.float32)).get()
+    assert list(got[0]) == [5.0, 8.5, 8.5]  # g2p_11
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(-2, 34, size=(5000, 3)).astype(np.float32)
+    for interp in (0, 1):
+        got = T.MacSampler(mac, pkg.BoundaryCondition.Clamp).sample(pos, interp).get()
+        want = np.zeros((5000, 3), np.float32)
+        orc.orc_mac_sample(32, 32, 32, slabs, slabs.shape[1], b2s, pos, 5000, interp, 0, want)
+        assert_bitexact(got, want, "mac sample")
+    uo, u, ut = (rng.normal(size=(5000, 3)).astype(np.float32) for _ in range(3))
+    for alpha in (0.0, 0.95, 1.0):
+        got = T.g2p_blend(uo, u, ut, alpha, lib=hip).get()
+        want = np.zeros_like(uo)
+        orc.orc_g2p_blend(uo, u, ut, 5000, alpha, want)
+        assert_bitexact(got, want, "blend")
+
+
+def mac_bound(orc, s, pos, vel, mass, r_p):
+    """Per-cell sum of |contributions| (mass, |momentum|) for the tolerance."""
+    _, b2s, slabs = oracle_p2g_mac(orc, s, pos, np.abs(vel), mass, r_p)
+    return b2s, slabs
+
+
+@pytest.mark.parametrize("n,s,r_p", [(1, (32, 32, 32), 2.0), (3000, (48, 48, 48), 2.0), (20000, (64, 48, 80), 1.5),
+                                     (4000, (40, 33, 20), 2.0), (5000, (48, 48, 48), 3.0)])
+def test_particles_to_grid_mac_matches_oracle(pkg, orc, hip, n, s, r_p):
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(n, 3)).astype(np.float32)  # inside [0, s)
+    if n == 1:
+        pos[0] = [15.9, 8.0, 8.0]
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    active, b2s, slabs = oracle_p2g_mac(orc, s, pos, vel, mass, r_p)
+    mac = T.MacGrid(*s, capacity_blocks=active.size + 8, lib=hip)
+    k = T.particles_to_grid(mac, pos, vel, mass, r_p)
+    assert k == active.size
+    b2s_abs, slabs_abs = mac_bound(orc, s, pos, vel, mass, r_p)
+    for a in range(3):
+        m_dev = mac.mass[a].to_host()
+        v_dev = mac.velocity[a].to_host()
+        assert np.array_equal(m_dev.b2s >= 2, b2s >= 2)
+        M = m_dev.to_dense()
+        Mo = dense_from_slabs(b2s, slabs[3 + a], s)
+        # Order-dependent float32 sums (LDS atomics vs the oracle's particle
+        # order): all mass terms are positive, so |dM| <= 2e-6 * M.
+        assert np.all(np.abs(M - Mo) <= 2e-6 * Mo + 1e-30), np.abs(M - Mo).max()
+        V = v_dev.to_dense()
+        Vo = dense_from_slabs(b2s, slabs[a], s)
+        Pabs = dense_from_slabs(b2s_abs, slabs_abs[a], s) * Mo  # sum of |m w u|
+        # |dP| <= 2e-6 * sum|m w u| and |dM| <= 2e-6 M, so |dV| <= 2e-6 (sum|m w u| / M + |V|).
+        bound = 2e-6 * (np.where(Mo > 0, Pabs / np.where(Mo > 0, Mo, 1), 0) + np.abs(Vo)) + 1e-30
+        assert np.all(np.abs(V - Vo) <= bound), np.max(np.abs(V - Vo) - bound)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("n,s", [(1, (32, 32, 32)), (20000, (64, 48, 80)), (200000, (128, 128, 128))])
+def test_bspline_p2g_g2p_bitexact(pkg, orc, hip, order, n, s):
+    rng = np.random.default_rng(n + order)
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(n, 3)).astype(np.float32)  # inside [0, s)
+    if n > 100:
+        pos[50:80] = pos[10]  # several particles in one voxel
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32) if n != 20000 else None
+    active, b2s, slabs = oracle_p2g_bspline(orc, order, s, pos, vel, mass)
+    tg = T.TransferGrid(*s, capacity_blocks=active.size + 4, lib=hip)
+    assert tg.p2g(order, pos, vel, mass) == active.size
+    got_m = tg.mass.to_host()
+    assert np.array_equal(got_m.b2s >= 2, b2s >= 2)
+    assert_bitexact(got_m.to_dense(), dense_from_slabs(b2s, slabs[0], s), "mass")
+    for a in range(3):
+        assert_bitexact(tg.velocity[a].to_host().to_dense(), dense_from_slabs(b2s, slabs[1 + a], s), "vel %d" % a)
+    out = tg.g2p(order, pos).get()
+    want = oracle_g2p_bspline(orc, order, s, b2s, slabs[1:], pos)
+    assert_bitexact(out, want, "g2p")

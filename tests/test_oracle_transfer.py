@@ -1,0 +1,179 @@
+"""Pins the oracle's transfers: the reference's P2G/G2P unit tests
+(fluid/p2g.rs:527-619, fluid/g2p.rs:62-129) and closed forms for the
+B-spline transfers that have no reference counterpart."""
+import numpy as np
+import pytest
+
+from transfer_helpers import (dense_from_slabs, oracle_g2p_bspline, oracle_p2g_bspline, oracle_p2g_mac,
+                              scatter_bspline_f64)
+
+S32 = (32, 32, 32)
+
+
+def naive(orc, s, pos, vel, mass, r_p):
+    sx, sy, sz = s
+    d = np.zeros((6, sz, sy, sx), np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    orc.orc_p2g_mac_naive(sx, sy, sz, pos, np.ascontiguousarray(vel, np.float32),
+                          np.ascontiguousarray(mass, np.float32), pos.shape[0], r_p, d)
+    return d
+
+
+def staged_dense(orc, s, pos, vel, mass, r_p, ref_commit_width=0):
+    active, b2s, slabs = oracle_p2g_mac(orc, s, pos, vel, mass, r_p, ref_commit_width)
+    return np.stack([dense_from_slabs(b2s, slabs[c], s) for c in range(6)])
+
+
+@pytest.mark.parametrize("pos,vel", [([[8.0, 8.0, 8.0]], [[1.0, 2.0, 3.0]]),     # p2g_01
+                                     ([[15.9, 8.0, 8.0]], [[1.0, 0.0, 0.0]]),    # p2g_03
+                                     ([[0.1, 0.1, 0.1]], [[1.0, 1.0, 1.0]])])    # p2g_04
+def test_staged_p2g_matches_naive_reference_tests(orc, pos, vel):
+    a = staged_dense(orc, S32, pos, vel, [1.0], 2.0)
+    b = naive(orc, S32, pos, vel, [1.0], 2.0)
+    np.testing.assert_allclose(a, b, atol=1e-4, rtol=0)
+    if pos[0][0] == 15.9:
+        assert (a[3, 8, 8, 16:] > 0).any()  # mass reached the +x neighbour block
+
+
+def test_staged_p2g_many_particles_matches_naive(orc):
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(0.5, 47.5, size=(400, 3)).astype(np.float32)
+    vel = rng.normal(size=(400, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, size=400).astype(np.float32)
+    s = (48, 48, 48)
+    a = staged_dense(orc, s, pos, vel, mass, 2.0)
+    b = naive(orc, s, pos, vel, mass, 2.0)
+    np.testing.assert_allclose(a[3:], b[3:], rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(a[:3], b[:3], rtol=1e-4, atol=1e-4)
+
+
+def test_reference_margin_commit_spill(orc):
+    """The reference commits 4 lanes of a 2-wide -x margin region (p2g.rs:396-406):
+    for a particle in a block with bx >= 1 near its -x face the staged splat then
+    adds centre-block staging columns to the next row of the -x neighbour block,
+    cells no particle reaches.  ref_commit_width = 1 reproduces it; the naive
+    oracle (p2g.rs:445-502) and the exact-width commit do not."""
+    pos, vel = [[16.6, 8.0, 8.0]], [[1.0, 0.0, 0.0]]
+    quirk = staged_dense(orc, S32, pos, vel, [1.0], 2.0, ref_commit_width=1)
+    exact = staged_dense(orc, S32, pos, vel, [1.0], 2.0, ref_commit_width=0)
+    ref = naive(orc, S32, pos, vel, [1.0], 2.0)
+    np.testing.assert_allclose(exact, ref, atol=1e-5, rtol=0)
+    diff = np.argwhere(np.abs(quirk[3] - ref[3]) > 1e-6)
+    assert diff.size > 0
+    # every spilled cell is at x in {0, 1} of block bx = 0, one row (y + 1) above a touched row
+    assert set(diff[:, 2]) <= {0, 1}
+
+
+def test_rigid_translation_roundtrip(orc):
+    """p2g_02: a rigidly moving cloud samples back to its velocity (Linear, Clamp)."""
+    u = np.array([1.5, -2.0, 0.5], np.float32)
+    pos = np.array([[8.0, 8.0, 8.0], [12.0, 9.0, 7.0], [9.0, 14.0, 10.0]], np.float32)
+    active, b2s, slabs = oracle_p2g_mac(orc, S32, pos, np.tile(u, (3, 1)), np.ones(3, np.float32), 2.0)
+    out = np.zeros((3, 3), np.float32)
+    orc.orc_mac_sample(32, 32, 32, np.ascontiguousarray(slabs[:3]), slabs.shape[1], b2s, pos, 3, 0, 0, out)
+    np.testing.assert_allclose(out, np.tile(u, (3, 1)), atol=1e-4)
+
+
+def test_radius_cull_and_zero_mass_guard(orc):
+    """p2g_05 / p2g_07."""
+    d = staged_dense(orc, S32, [[8.0, 8.0, 8.0]], [[1.0, 0.0, 0.0]], [1.0], 2.0)
+    assert d[3, 8, 8, 10] == 0.0 and d[3, 8, 8, 7] > 0.0
+    assert d[0, 20, 20, 20] == 0.0 and np.isfinite(d[0, 8, 8, 7])
+
+
+def test_overlap_sum_order_independent(orc):
+    """p2g_06."""
+    a = staged_dense(orc, S32, [[8.0, 8.0, 8.0], [9.0, 8.0, 8.0]], [[1.0, 0.0, 0.0]] * 2, [1.0, 1.0], 2.0)
+    b = staged_dense(orc, S32, [[9.0, 8.0, 8.0], [8.0, 8.0, 8.0]], [[1.0, 0.0, 0.0]] * 2, [1.0, 1.0], 2.0)
+    np.testing.assert_allclose(a, b, atol=1e-5)
+
+
+def face_field_slabs(b2s_n=8):
+    """g2p_11 grid: u(i,j,k) = i+1, v = j+1, w = k+1 over a 32^3 domain."""
+    b2s = np.arange(2, 2 + b2s_n, dtype=np.int32)
+    slabs = np.zeros((3, 2 + b2s_n, 4096), np.float32)
+    v = np.arange(4096)
+    lx, ly, lz = v & 15, (v >> 4) & 15, v >> 8
+    for bid in range(8):
+        bx, by, bz = bid % 2, (bid // 2) % 2, bid // 4
+        slabs[0, 2 + bid] = bx * 16 + lx + 1
+        slabs[1, 2 + bid] = by * 16 + ly + 1
+        slabs[2, 2 + bid] = bz * 16 + lz + 1
+    return b2s, slabs
+
+
+def test_g2p_face_center_exact(orc):
+    """g2p_11: sampling at a face centre returns that face value exactly."""
+    b2s, slabs = face_field_slabs()
+    out = np.zeros((1, 3), np.float32)
+    orc.orc_mac_sample(32, 32, 32, slabs, slabs.shape[1], b2s, np.array([[5.0, 8.5, 8.5]], np.float32), 1, 0, 0, out)
+    assert list(out[0]) == [5.0, 8.5, 8.5]
+
+
+def test_g2p_blend_endpoints(orc):
+    """g2p_08: alpha 0 = PIC (u), alpha 1 = FLIP (u_old + u - u_tilde)."""
+    uo, u, ut = (np.array([[1, 2, 3]], np.float32), np.array([[4, 5, 6]], np.float32),
+                 np.array([[4.5, 4.5, 4.5]], np.float32))
+    out = np.zeros((1, 3), np.float32)
+    orc.orc_g2p_blend(uo, u, ut, 1, 0.0, out)
+    assert np.array_equal(out, u)
+    orc.orc_g2p_blend(uo, u, ut, 1, 1.0, out)
+    assert np.array_equal(out, uo + (u - ut))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_bspline_partition_of_unity_and_closed_form(orc, order):
+    rng = np.random.default_rng(order)
+    for p in rng.uniform(2, 30, 200).astype(np.float32):
+        f = int(np.floor(p))
+        w = [orc.orc_bspline_weight(order, float(p), c) for c in range(f - 2, f + 3)]
+        assert abs(sum(w) - 1.0) < 2e-7
+        x = float(np.float32(p) - np.float32(0.5)) - f  # distance to cell f's centre
+        if order == 2:
+            assert abs(w[2] - (0.75 - x * x)) < 1e-6  # quadratic B-spline 3/4 - x^2
+        else:
+            assert abs(w[2] - max(0.0, 1.0 - abs(x))) < 1e-6
+    assert orc.orc_bspline_weight(2, 10.5, 10) == 0.75 and orc.orc_bspline_weight(2, 10.5, 11) == 0.125
+    assert orc.orc_bspline_weight(1, 10.5, 10) == 1.0 and orc.orc_bspline_weight(1, 10.5, 11) == 0.0
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_bspline_p2g_conservation_and_scatter_equivalence(orc, order):
+    rng = np.random.default_rng(10 + order)
+    s = (48, 40, 32)
+    pos = rng.uniform(2.0, [46.0, 38.0, 30.0], size=(300, 3)).astype(np.float32)
+    vel = rng.normal(size=(300, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, 300).astype(np.float32)
+    active, b2s, slabs = oracle_p2g_bspline(orc, order, s, pos, vel, mass)
+    M = dense_from_slabs(b2s, slabs[0], s).astype(np.float64)
+    V = np.stack([dense_from_slabs(b2s, slabs[1 + a], s) for a in range(3)]).astype(np.float64)
+    assert abs(M.sum() - mass.astype(np.float64).sum()) < 1e-4  # mass conservation
+    mom = (M[None] * V).reshape(3, -1).sum(1)
+    np.testing.assert_allclose(mom, (mass[:, None] * vel).astype(np.float64).sum(0), atol=1e-3)  # momentum
+    Mb, Pb = scatter_bspline_f64(orc, order, s, pos, vel, mass)
+    np.testing.assert_allclose(M, Mb, rtol=1e-5, atol=1e-6)
+    Vb = np.where(Mb > 0, Pb / np.where(Mb > 0, Mb, 1), 0)
+    np.testing.assert_allclose(V, Vb, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_bspline_g2p_reproduces_affine_field(orc, order):
+    """B-splines of order 1 and 2 reproduce affine functions exactly."""
+    s = (48, 48, 48)
+    nb = 27
+    b2s = np.arange(2, 2 + nb, dtype=np.int32)
+    slabs = np.zeros((3, 2 + nb, 4096), np.float32)
+    v = np.arange(4096)
+    for bid in range(nb):
+        bx, by, bz = bid % 3, (bid // 3) % 3, bid // 9
+        cx, cy, cz = bx * 16 + (v & 15) + 0.5, by * 16 + ((v >> 4) & 15) + 0.5, bz * 16 + (v >> 8) + 0.5
+        slabs[0, 2 + bid] = 0.5 * cx - 0.25 * cy + 2.0
+        slabs[1, 2 + bid] = 0.1 * cz + 1.0
+        slabs[2, 2 + bid] = -0.3 * cx + 0.2 * cy + 0.1 * cz
+    rng = np.random.default_rng(order)
+    pos = rng.uniform(3, 45, size=(500, 3)).astype(np.float32)
+    out = oracle_g2p_bspline(orc, order, s, b2s, slabs, pos)
+    p = pos.astype(np.float64)
+    want = np.stack([0.5 * p[:, 0] - 0.25 * p[:, 1] + 2.0, 0.1 * p[:, 2] + 1.0,
+                     -0.3 * p[:, 0] + 0.2 * p[:, 1] + 0.1 * p[:, 2]], 1)
+    np.testing.assert_allclose(out, want, atol=2e-5)
