@@ -81,6 +81,60 @@ def cpu_baseline(sc, dt_mc, dt_bl, reps):
                       "best of %d after 1 warm-up, FTZ/DAZ off" % (active.size, reps)}
 
 
+def particle_leg(pkg, lib, sc, n, steps, warmup):
+    """16M particles uniform in the level-0 band: stable 30-bit Morton sort,
+    then quadratic B-spline P2G (sort + voxel ranges + gather) and G2P, each
+    timed with HIP events around its call (inputs resident in HBM)."""
+    import ctypes as C
+    from msbg_rs_amd import scenes
+    from msbg_rs_amd import transfer as T
+
+    t = time.time()
+    pos, vel = scenes.band_particles(sc, n)
+    log("particles: %d generated in %.1fs" % (n, time.time() - t))
+    dpos, dvel = lib.to_device(pos), lib.to_device(vel)
+    cap = 3 * int((sc.levels == 0).sum()) + 4096
+    tg = T.TransferGrid(sc.S, sc.S, sc.S, capacity_blocks=cap, lib=lib)
+    keys, perm = lib.empty(n, np.uint32), lib.empty(n, np.uint32)
+    out = lib.empty((n, 3), np.float32)
+    handles = (C.c_void_p * 3)(*[g.h for g in tg.velocity])
+    k = C.c_int64()
+
+    def p2g():
+        lib.call("msbg_p2g_bspline", 2, tg.mass.h, handles, dpos.ptr, dvel.ptr, None, n, C.byref(k))
+
+    def g2p():
+        lib.call("msbg_g2p_bspline", 2, handles, dpos.ptr, n, out.ptr)
+
+    def sort():
+        lib.call("msbg_morton_sort", dpos.ptr, n, keys.ptr, perm.ptr)
+
+    for _ in range(warmup):
+        sort()
+        p2g()
+        g2p()
+    ms = {"sort": 0.0, "p2g": 0.0, "g2p": 0.0}
+    e0, e1 = lib.event(), lib.event()
+    for _ in range(steps):
+        for name, fn in (("sort", sort), ("p2g", p2g), ("g2p", g2p)):
+            lib.call("msbg_event_record", e0)
+            fn()
+            lib.call("msbg_event_record", e1)
+            ms[name] += lib.elapsed_ms(e0, e1)
+    for key in ms:
+        ms[key] /= steps
+    step_ms = ms["p2g"] + ms["g2p"]
+    return {
+        "particles": n,
+        "active_blocks": k.value,
+        "ms": ms,
+        "morton_sort_particles_per_s": n / (ms["sort"] * 1e-3),
+        "p2g_g2p_particles_per_s": n / (step_ms * 1e-3),
+        "note": "quadratic B-spline P2G (includes its own Morton sort, active-block build and voxel ranges) "
+                "+ G2P, cell-centred 16^3-block grid over the 512^3 domain",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +142,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--particles", type=int, default=16 * 1024 * 1024)
+    ap.add_argument("--particle-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the sparse-grid path is not sharded across GPUs "
@@ -181,6 +237,9 @@ def main():
         },
         "kernel_ms": {"mean_curvature": k_ms[1], "bilaplacian": k_ms[2]},
     }
+    if args.particles > 0:
+        log("particle leg ...")
+        out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(sc, dt_mc, dt_bl, args.cpu_reps)

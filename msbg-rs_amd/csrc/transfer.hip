@@ -272,14 +272,26 @@ __device__ __forceinline__ float bspline_w(float p, int ci) {
     return 0.0f;
 }
 
+// Sorted particle records for the B-spline P2G: pm = (x, y, z, m), pu = (u, v, w, 0).
+__global__ void gather_payload4_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
+                                       const float *__restrict__ mass, const uint32_t *__restrict__ perm, int64_t n,
+                                       float4 *__restrict__ pm, float4 *__restrict__ pu) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = perm[i];
+        pm[i] = make_float4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass ? mass[j] : 1.0f);
+        pu[i] = make_float4(vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], 0.0f);
+    }
+}
+
 // Voxel ranges of the sorted particles: vbeg/vend indexed by slot*4096+vid.
-__global__ void voxel_ranges_kernel(const uint32_t *__restrict__ keys, const float *__restrict__ spos, int64_t n,
+__global__ void voxel_ranges_kernel(const uint32_t *__restrict__ keys, const float4 *__restrict__ pm, int64_t n,
                                     GridView g, uint32_t *__restrict__ vbeg, uint32_t *__restrict__ vend) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const bool head = i == 0 || keys[i - 1] != keys[i];
         const bool tail = i == n - 1 || keys[i + 1] != keys[i];
         if (!head && !tail) continue;
-        const uint32_t x = trunc_index(spos[3 * i]), y = trunc_index(spos[3 * i + 1]), z = trunc_index(spos[3 * i + 2]);
+        const float4 p = pm[i];
+        const uint32_t x = trunc_index(p.x), y = trunc_index(p.y), z = trunc_index(p.z);
         if (x >= (uint32_t)g.sx || y >= (uint32_t)g.sy || z >= (uint32_t)g.sz) continue;
         const int slot = g.b2s[(x >> 4) + (int64_t)(y >> 4) * g.nx + (int64_t)(z >> 4) * g.nx * g.ny];
         if (slot < 2) continue;
@@ -289,17 +301,32 @@ __global__ void voxel_ranges_kernel(const uint32_t *__restrict__ keys, const flo
     }
 }
 
-// Workgroup per active block, 256 lanes x 16 cells (z walk).  Each cell sums
-// its 27 neighbour voxels (z, y, x from -1 to 1) and each voxel's particles in
-// sorted order, exactly as the oracle does.
+struct Acc4 {
+    float m, u0, u1, u2;
+};
+
+template <int ORDER>
+__device__ __forceinline__ void acc_add(Acc4 &a, float wxy, float pz, int cz, float m, const float4 &u) {
+    const float wm = (wxy * bspline_w<ORDER>(pz, cz)) * m;
+    a.m = a.m + wm;
+    a.u0 = a.u0 + wm * u.x;
+    a.u1 = a.u1 + wm * u.y;
+    a.u2 = a.u2 + wm * u.z;
+}
+
+// Workgroup per active block; lane (lx, ly) owns the cell column (lx, ly, *).
+// A cell sums its 27 neighbour voxels, z then y then x from -1 to 1, and each
+// voxel's particles in sorted order -- the oracle's order.  The lane walks the
+// 18 voxel planes bz*16-1 .. bz*16+16 once: plane v feeds cells v+1 (as their
+// dz = -1 plane), v (dz = 0) and v-1 (dz = +1), so each particle is read once
+// per column and every cell still receives its planes in increasing z.
 template <int ORDER>
 __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView v0, GridView v1, GridView v2,
                                                           const int32_t *__restrict__ active,
                                                           const uint32_t *__restrict__ vbeg,
                                                           const uint32_t *__restrict__ vend,
-                                                          const float *__restrict__ spos,
-                                                          const float *__restrict__ svel,
-                                                          const float *__restrict__ smass) {
+                                                          const float4 *__restrict__ pm,
+                                                          const float4 *__restrict__ pu) {
     __shared__ int s_nb[27];
     const int64_t bid = active[blockIdx.x];
     const int bx = (int)(bid % mg.nx), by = (int)((bid / mg.nx) % mg.ny), bz = (int)(bid / ((int64_t)mg.nx * mg.ny));
@@ -315,49 +342,54 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
     const int s0 = v0.b2s[bid], s1 = v1.b2s[bid], s2 = v2.b2s[bid];
     if (ms < 2) return;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int cx = bx * 16 + lx, cy = by * 16 + ly;
-    for (int lz = 0; lz < 16; ++lz) {
-        const int cz = bz * 16 + lz;
-        float am = 0.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int vz = cz + dz;
-            if (vz < 0 || vz >= mg.sz) continue;
+    const int cx = bx * 16 + lx, cy = by * 16 + ly, cz0 = bz * 16;
+    Acc4 lo = {0, 0, 0, 0}, mid = {0, 0, 0, 0}, hi = {0, 0, 0, 0};  // cells lv-1, lv, lv+1
+    auto store = [&](int lz, const Acc4 &a) {
+        const int vid = lx | (ly << 4) | (lz << 8);
+        mg.slab[(int64_t)ms * 4096 + vid] = a.m;
+        const bool pos_m = a.m > 0.0f;
+        v0.slab[(int64_t)s0 * 4096 + vid] = pos_m ? a.u0 / a.m : 0.0f;
+        v1.slab[(int64_t)s1 * 4096 + vid] = pos_m ? a.u1 / a.m : 0.0f;
+        v2.slab[(int64_t)s2 * 4096 + vid] = pos_m ? a.u2 / a.m : 0.0f;
+    };
+    for (int lv = -1; lv <= 16; ++lv) {
+        const int vz = cz0 + lv;
+        if (vz >= 0 && vz < mg.sz) {
             for (int dy = -1; dy <= 1; ++dy) {
                 const int vy = cy + dy;
                 if (vy < 0 || vy >= mg.sy) continue;
                 for (int dx = -1; dx <= 1; ++dx) {
                     const int vx = cx + dx;
                     if (vx < 0 || vx >= mg.sx) continue;
-                    const int nbi = ((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1);
-                    const int slot = s_nb[nbi];
+                    const int slot = s_nb[((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1)];
                     if (slot < 2) continue;
                     const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
                     const uint32_t e = vend[c];
                     for (uint32_t j = vbeg[c]; j < e; ++j) {
-                        const float w = (bspline_w<ORDER>(spos[3 * j], cx) * bspline_w<ORDER>(spos[3 * j + 1], cy)) *
-                                        bspline_w<ORDER>(spos[3 * j + 2], cz);
-                        const float wm = w * smass[j];
-                        am = am + wm;
-                        a0 = a0 + wm * svel[3 * j];
-                        a1 = a1 + wm * svel[3 * j + 1];
-                        a2 = a2 + wm * svel[3 * j + 2];
+                        const float4 p = pm[j];
+                        const float4 u = pu[j];
+                        const float wxy = bspline_w<ORDER>(p.x, cx) * bspline_w<ORDER>(p.y, cy);
+                        if (lv >= 1) acc_add<ORDER>(lo, wxy, p.z, cz0 + lv - 1, p.w, u);
+                        if (lv >= 0 && lv <= 15) acc_add<ORDER>(mid, wxy, p.z, cz0 + lv, p.w, u);
+                        if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
                     }
                 }
             }
         }
-        const int vid = lx | (ly << 4) | (lz << 8);
-        mg.slab[(int64_t)ms * 4096 + vid] = am;
-        const bool pos_m = am > 0.0f;
-        v0.slab[(int64_t)s0 * 4096 + vid] = pos_m ? a0 / am : 0.0f;
-        v1.slab[(int64_t)s1 * 4096 + vid] = pos_m ? a1 / am : 0.0f;
-        v2.slab[(int64_t)s2 * 4096 + vid] = pos_m ? a2 / am : 0.0f;
+        if (lv >= 1) store(lv - 1, lo);  // cell lv-1 has its planes lv-2, lv-1, lv
+        lo = mid;
+        mid = hi;
+        hi = Acc4{0, 0, 0, 0};
     }
 }
 
 template <int ORDER>
 __global__ void g2p_bspline_kernel(GridView v0, GridView v1, GridView v2, const float *__restrict__ pos, int64_t n,
-                                   float *__restrict__ out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+                                   const uint32_t *__restrict__ visit, float *__restrict__ out) {
+    // Lane k handles particle visit[k] (Morton order), so neighbouring lanes
+    // read neighbouring grid cells; each particle's sum is independent of it.
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = visit[k];
         const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
         const int fx = (int)floorf(px), fy = (int)floorf(py), fz = (int)floorf(pz);
         float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
@@ -588,11 +620,11 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
     uint32_t *perm = keys + n;
     rc = msbg_morton_sort(d_pos, n, keys, perm);
     if (rc) return rc;
-    float *pay = (float *)workspace(7, sizeof(float) * 7 * n, &rc);
+    float4 *pay = (float4 *)workspace(7, sizeof(float4) * 2 * n, &rc);
     if (rc) return rc;
-    float *spos = pay, *svel = pay + 3 * n, *smass = pay + 6 * n;
-    hipLaunchKernelGGL(gather_payload_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass, perm,
-                       n, spos, svel, smass);
+    float4 *pm = pay, *pu = pay + n;
+    hipLaunchKernelGGL(gather_payload4_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass,
+                       perm, n, pm, pu);
     MSBG_HIP(hipGetLastError());
     // per-voxel ranges of the sorted particles, indexed like the mass slab
     const int64_t ncell = mass->n_slots * 4096;
@@ -600,16 +632,16 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
     if (rc) return rc;
     uint32_t *vbeg = vr, *vend = vr + ncell;
     MSBG_HIP(hipMemsetAsync(vr, 0, sizeof(uint32_t) * 2 * ncell, stream()));
-    hipLaunchKernelGGL(voxel_ranges_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), keys, spos, n, mass->view(),
+    hipLaunchKernelGGL(voxel_ranges_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), keys, pm, n, mass->view(),
                        vbeg, vend);
     MSBG_HIP(hipGetLastError());
     const dim3 grid((unsigned)act.size()), block(256);
     if (order == 1)
         hipLaunchKernelGGL(p2g_bspline_kernel<1>, grid, block, 0, stream(), mass->view(), vel[0]->view(), vel[1]->view(),
-                           vel[2]->view(), d_act, vbeg, vend, spos, svel, smass);
+                           vel[2]->view(), d_act, vbeg, vend, pm, pu);
     else
         hipLaunchKernelGGL(p2g_bspline_kernel<2>, grid, block, 0, stream(), mass->view(), vel[0]->view(), vel[1]->view(),
-                           vel[2]->view(), d_act, vbeg, vend, spos, svel, smass);
+                           vel[2]->view(), d_act, vbeg, vend, pm, pu);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;
@@ -621,13 +653,19 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
     int rc = check_same_dims(vel, 3);
     if (rc) return rc;
     MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "g2p_bspline: null argument");
+    MSBG_CHECK(n < ((int64_t)1 << 31), MSBG_EINVAL, "g2p_bspline: n must be below 2^31");
     if (n == 0) return MSBG_OK;
+    uint32_t *keys = (uint32_t *)workspace(9, sizeof(uint32_t) * 2 * n, &rc);
+    if (rc) return rc;
+    uint32_t *visit = keys + n;
+    rc = msbg_morton_sort(d_pos, n, keys, visit);
+    if (rc) return rc;
     if (order == 1)
         hipLaunchKernelGGL(g2p_bspline_kernel<1>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
-                           vel[1]->view(), vel[2]->view(), d_pos, n, d_out);
+                           vel[1]->view(), vel[2]->view(), d_pos, n, visit, d_out);
     else
         hipLaunchKernelGGL(g2p_bspline_kernel<2>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
-                           vel[1]->view(), vel[2]->view(), d_pos, n, d_out);
+                           vel[1]->view(), vel[2]->view(), d_pos, n, visit, d_out);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;

@@ -182,6 +182,20 @@ def noisy_sphere_scene(S=512, n_levels=4, band=8.0, seed=4, amp=6.0, regularize=
     return MultiresScene(S, n_levels, levels, grids, sdf, c, R)
 
 
+def band_particles(scene: "MultiresScene", n, seed_pos=5, seed_vel=6):
+    """n particles uniform over the level-0 blocks of a multires scene (each
+    particle picks a block, then a position inside it) with N(0,1) float32
+    velocities; positions stay strictly inside the domain."""
+    blocks = scene.level_blocks(0)
+    nb = scene.S // 16
+    u = lcg_uniform(seed_pos, 4 * n).reshape(n, 4).astype(np.float64)
+    pick = blocks[np.minimum((u[:, 0] * blocks.size).astype(np.int64), blocks.size - 1)]
+    org = np.stack([pick % nb, (pick // nb) % nb, pick // (nb * nb)], 1) * 16.0
+    pos = (org + np.minimum(u[:, 1:] * 16.0, 16.0 - 1e-3)).astype(np.float32)
+    vel = lcg_normal(seed_vel, 3 * n).reshape(n, 3)
+    return pos, vel
+
+
 def reference_shell_blocks(nx, ny, nz):
     """generate_active_blocks (benches/allocator_benches.rs:233-256): blocks whose
     centre lies in the shell [0.9, 1.0] * min(n)/2."""
