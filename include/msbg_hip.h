@@ -225,6 +225,64 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
 /* The matching G2P: d_out (n x 3) = sum over the support cells of w * vel. */
 int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64_t n, float *d_out);
 
+/* ---- Poisson relaxation on a band of value blocks ------------------------
+ * No reference counterpart: the reference's solver is planned, not built
+ * (docs/roadmap.md:610-702; the weighted Jacobi formula y = omega (b + sum nb)
+ * D + (1 - omega) x is planned at docs/roadmap.md:622-626).  The unknowns are
+ * the cells of the listed 16^3 value blocks of a layout grid; every field is a
+ * device slab shaped like the layout grid's (n_slots * 4096 elements of the
+ * storage type, slot-major, slots 0 / 1 = values read for absent / full
+ * neighbour blocks).  Operator: (A x)_c = 6 x_c - sum of the 6 face
+ * neighbours, a neighbour outside the domain being the cell itself (Neumann
+ * mirror, math/boundary.rs:35-56).  One relaxation of cell c:
+ *   s = ((((x-x + x+x) + x-y) + x+y) + x-z) + x+z,  t = fmaf(-6, x_c, b_c + s),
+ *   x_c' = fmaf(omega / 6, t, x_c) rounded to the storage type (RNE).
+ * Storage types: FP32, IEEE binary16, bfloat16; arithmetic is FP32. */
+enum { MSBG_STORAGE_F32 = 0, MSBG_STORAGE_F16 = 1, MSBG_STORAGE_BF16 = 2 };
+typedef struct msbg_poisson msbg_poisson;
+typedef struct {
+    int storage;          /* MSBG_STORAGE_*: storage the sweeps read and write */
+    int refine;           /* 0: red-black sweeps on x itself (storage must be F32);
+                             1: FP32 iterative refinement: r = b - A x (FP32), rt = r / 2^k
+                                (2^k ~ rms(r)), e = 0, inner sweeps on A e = rt in `storage`,
+                                x = fmaf(2^k, e, x) */
+    float omega;          /* SOR factor, 0 < omega < 2 */
+    int sweeps_per_check; /* red-black sweeps between FP64 residual checks (inner sweeps per refinement step) */
+    int max_sweeps;
+    int reserved;         /* 0 */
+    double tol;           /* stop when ||b - A x||_2 <= tol * ||b||_2 (FP64 norms) */
+} msbg_poisson_params;
+typedef struct {
+    int sweeps;           /* red-black sweeps done */
+    int checks;           /* residual evaluations */
+    int converged;        /* 1 when the tolerance was met */
+    int reserved;
+    double rel_residual;  /* ||b - A x|| / ||b|| at the last check */
+    double r2, b2;        /* squared norms at the last check */
+    float ms;             /* wall time of the solve on the library stream (HIP events) */
+} msbg_poisson_stats;
+
+int msbg_poisson_new(const msbg_grid *layout, const int64_t *active_blocks, int64_t n_active, msbg_poisson **out);
+int msbg_poisson_free(msbg_poisson *p);
+int msbg_poisson_info(const msbg_poisson *p, int64_t *n_active, int64_t *n_cells, int64_t *n_slots);
+/* One weighted Jacobi sweep x_out = x_in + omega/6 (b - A x_in) over the listed
+ * cells (x_in != x_out; other cells of x_out are not written). */
+int msbg_poisson_jacobi(msbg_poisson *p, int storage, const void *x_in, void *x_out, const void *b, float omega);
+/* One red-black SOR half sweep in place over the listed cells with
+ * (x + y + z) & 1 == color. */
+int msbg_poisson_rbgs(msbg_poisson *p, int storage, void *x, const void *b, int color, float omega);
+/* r = b - A x over the listed cells (r: FP32 slab or NULL) and the FP64 squared
+ * norms of r and b (synchronous). */
+int msbg_poisson_residual(msbg_poisson *p, int storage, const void *x, const void *b, float *r, double *r2,
+                          double *b2);
+/* Element-wise storage conversion of n_elems values (RNE on narrowing). */
+int msbg_poisson_convert(int src_storage, const void *src, int dst_storage, void *dst, int64_t n_elems);
+/* Solve A x = b (x, b: FP32 slabs; x holds the initial guess) with red-black
+ * SOR, checking the FP64 relative residual every sweeps_per_check sweeps.  Each
+ * check period runs as one replayed HIP graph. */
+int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *params, float *x, const float *b,
+                       msbg_poisson_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

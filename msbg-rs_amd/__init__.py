@@ -13,3 +13,4 @@ from .device import (  # noqa: F401
     BoundaryCondition, DeviceGrid, PdeParams, Stencil, SweepPlan, Sweeper, apply_pde, color_buckets,
     downsample_channel_avg, halo_fill, halo_fill_multires, halo_topology, regularize_refinement_map, sweep_multires,
 )
+from .solve import PoissonBand, Storage  # noqa: F401

@@ -1,0 +1,642 @@
+// Poisson relaxation on the band of a grid's value blocks: red-black SOR and
+// weighted Jacobi sweeps of the 7-point operator with FP32 / FP16 / BF16 block
+// storage and FP32 arithmetic, the residual with FP64 norm reductions, and the
+// solve loop (plain FP32, or FP32 iterative refinement around low-precision
+// inner sweeps).
+//
+// The reference has no solver (docs/roadmap.md:610-702 "HAVEN'T STARTED"; the
+// Jacobi formula is planned at docs/roadmap.md:622-626).  Operator and
+// rounding are defined here:
+//   (A x)_c = 6 x_c - (x_{-x} + x_{+x} + x_{-y} + x_{+y} + x_{-z} + x_{+z})
+// over the cells of the listed value blocks; a neighbour outside the domain is
+// the cell itself (Neumann mirror, math/boundary.rs:35-56), a neighbour in an
+// absent / full block reads slot 0 / 1 of the field (the grid's dummy slots),
+// a neighbour in an unlisted value block reads its stored value (Dirichlet).
+// One relaxation of cell c:
+//   s = ((((x_{-x} + x_{+x}) + x_{-y}) + x_{+y}) + x_{-z}) + x_{+z}
+//   t = fmaf(-6, x_c, b_c + s)            (= (b - A x)_c)
+//   x_c' = fmaf(omega / 6, t, x_c)        rounded to the storage type (RNE)
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace msbg {
+namespace {
+
+using i64 = int64_t;
+
+template <int ST>
+struct Sto;
+template <>
+struct Sto<MSBG_STORAGE_F32> {
+    using T = float;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return p[i]; }
+    static __device__ __forceinline__ T cv(float v) { return v; }
+};
+template <>
+struct Sto<MSBG_STORAGE_F16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
+    // The FP32 value is materialised before the RNE narrowing: without the
+    // register barrier the compiler fuses fmaf + cvt into one mixed-precision
+    // FMA (v_fma_mixlo_f16), which rounds once to f16 instead of f32 then f16.
+    static __device__ __forceinline__ T cv(float v) {
+        asm volatile("" : "+v"(v));
+        return __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+};
+template <>
+struct Sto<MSBG_STORAGE_BF16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return __uint_as_float((uint32_t)p[i] << 16); }
+    static __device__ __forceinline__ T cv(float v) {
+        uint32_t u = __float_as_uint(v);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (T)(u >> 16);
+    }
+};
+
+int elem_bytes(int storage) { return storage == MSBG_STORAGE_F32 ? 4 : 2; }
+
+// One listed value block: its slot, voxel origin and the slots of its six
+// face neighbours (-x, +x, -y, +y, -z, +z; unused where the face is on the
+// domain boundary).
+struct PBlock {
+    int32_t slot, ox, oy, oz;
+    int32_t nb[6];
+};
+
+struct Dom {
+    int32_t sx, sy, sz;
+};
+
+// A work-group covers 4 z-slices of one block: 256 threads = 16 x 16 columns.
+constexpr int kThreads = 256;
+constexpr int kSlices = 4;
+
+struct Cell {
+    i64 c, xl, xr, yd, yu, zb, zf;
+};
+
+__device__ __forceinline__ Cell cell_at(const PBlock &B, const Dom &d, int tx, int ty, int tz, int gx, int gy, int gz) {
+    Cell k;
+    const int vid = tx | ty << 4 | tz << 8;
+    k.c = ((i64)B.slot << 12) + vid;
+    k.xl = tx > 0 ? k.c - 1 : (gx == 0 ? k.c : ((i64)B.nb[0] << 12) + (vid | 15));
+    k.xr = gx + 1 >= d.sx ? k.c : (tx < 15 ? k.c + 1 : ((i64)B.nb[1] << 12) + (vid & ~15));
+    k.yd = ty > 0 ? k.c - 16 : (gy == 0 ? k.c : ((i64)B.nb[2] << 12) + (vid | 0xf0));
+    k.yu = gy + 1 >= d.sy ? k.c : (ty < 15 ? k.c + 16 : ((i64)B.nb[3] << 12) + (vid & ~0xf0));
+    k.zb = tz > 0 ? k.c - 256 : (gz == 0 ? k.c : ((i64)B.nb[4] << 12) + (vid | 0xf00));
+    k.zf = gz + 1 >= d.sz ? k.c : (tz < 15 ? k.c + 256 : ((i64)B.nb[5] << 12) + (vid & ~0xf00));
+    return k;
+}
+
+template <int ST>
+__device__ __forceinline__ float residual_of(const typename Sto<ST>::T *x, const typename Sto<ST>::T *b, const Cell &k,
+                                             float &xc) {
+    using S = Sto<ST>;
+    xc = S::ld(x, k.c);
+    const float s = ((((S::ld(x, k.xl) + S::ld(x, k.xr)) + S::ld(x, k.yd)) + S::ld(x, k.yu)) + S::ld(x, k.zb)) +
+                    S::ld(x, k.zf);
+    return __builtin_fmaf(-6.0f, xc, S::ld(b, k.c) + s);
+}
+
+// RB: in place, cells with (gx + gy + gz) & 1 == color (their neighbours all
+// have the other colour, so a launch never reads a cell it writes).
+// !RB: weighted Jacobi, x_in -> x_out (distinct buffers).
+template <int ST, bool RB>
+__global__ __launch_bounds__(kThreads) void relax_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                         const typename Sto<ST>::T *xin, typename Sto<ST>::T *xout,
+                                                         const typename Sto<ST>::T *b, float w6, int color) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    if (gx >= d.sx || gy >= d.sy) return;
+    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+#pragma unroll
+    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+        const int gz = B.oz + tz;
+        if (gz >= d.sz) break;
+        if (RB && ((gx + gy + gz) & 1) != color) continue;
+        const Cell k = cell_at(B, d, tx, ty, tz, gx, gy, gz);
+        float xc;
+        const float t = residual_of<ST>(xin, b, k, xc);
+        xout[k.c] = Sto<ST>::cv(__builtin_fmaf(w6, t, xc));
+    }
+}
+
+// Residual r = b - A x (r optional, FP32) and per-work-group FP64 partial sums
+// of r^2 and b^2 (fixed-order reduction: lane butterfly, then waves 0..3).
+template <int ST>
+__global__ __launch_bounds__(kThreads) void residual_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                            const typename Sto<ST>::T *x,
+                                                            const typename Sto<ST>::T *b, float *r,
+                                                            double2 *partial) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    double rr = 0.0, bb = 0.0;
+    if (gx < d.sx && gy < d.sy) {
+        const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+        for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+            const int gz = B.oz + tz;
+            if (gz >= d.sz) break;
+            const Cell k = cell_at(B, d, tx, ty, tz, gx, gy, gz);
+            float xc;
+            const float t = residual_of<ST>(x, b, k, xc);
+            if (r) r[k.c] = t;
+            const double bv = Sto<ST>::ld(b, k.c);
+            rr += (double)t * (double)t;
+            bb += bv * bv;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rr += __shfl_xor(rr, o);
+        bb += __shfl_xor(bb, o);
+    }
+    __shared__ double s_rr[kThreads / 64], s_bb[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s_rr[threadIdx.x >> 6] = rr;
+        s_bb[threadIdx.x >> 6] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+            a += s_rr[w];
+            c += s_bb[w];
+        }
+        partial[blockIdx.x] = make_double2(a, c);
+    }
+}
+
+// Sums the partials (thread t takes t, t + 256, ..., then a fixed tree) into
+// norm[0] = ||r||^2, norm[1] = ||b||^2, and sets the refinement scale
+// scale[0] = 2^e ~ rms(r) (exact power of two, 1 when r = 0), scale[1] = 1 / scale[0].
+__global__ __launch_bounds__(256) void reduce_norms_kernel(const double2 *__restrict__ partial, int n, double n_cells,
+                                                           double *norm, float *scale) {
+    __shared__ double s_a[256], s_b[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partial[i].x;
+        b += partial[i].y;
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_a[threadIdx.x] += s_a[threadIdx.x + o];
+            s_b[threadIdx.x] += s_b[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        norm[0] = s_a[0];
+        norm[1] = s_b[0];
+        int e = 0;
+        const double rms = n_cells > 0 ? sqrt(s_a[0] / n_cells) : 0.0;
+        if (rms > 0.0 && isfinite(rms)) {
+            frexp(rms, &e);  // rms = m 2^e, m in [0.5, 1)
+            e -= 1;
+            e = e < -120 ? -120 : (e > 120 ? 120 : e);
+        }
+        scale[0] = ldexpf(1.0f, e);
+        scale[1] = ldexpf(1.0f, -e);
+    }
+}
+
+// Refinement step start: rt = T(r / scale), e = 0 on the listed cells.
+template <int ST>
+__global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__restrict__ blocks, Dom d, const float *r,
+                                                            const float *scale, typename Sto<ST>::T *rt,
+                                                            typename Sto<ST>::T *e) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    if (gx >= d.sx || gy >= d.sy) return;
+    const float inv = scale[1];
+    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+        if (B.oz + tz >= d.sz) break;
+        const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);
+        rt[c] = Sto<ST>::cv(r[c] * inv);
+        e[c] = Sto<ST>::cv(0.0f);
+    }
+}
+
+// Refinement step end: x = fmaf(scale, e, x) on the listed cells.
+template <int ST>
+__global__ __launch_bounds__(kThreads) void ir_update_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                             const typename Sto<ST>::T *e, const float *scale,
+                                                             float *x) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    if (gx >= d.sx || gy >= d.sy) return;
+    const float s = scale[0];
+    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+        if (B.oz + tz >= d.sz) break;
+        const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);
+        x[c] = __builtin_fmaf(s, Sto<ST>::ld(e, c), x[c]);
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Storage conversion of whole slabs (RNE on narrowing).
+template <int SS, int DS>
+__global__ void convert_kernel(const typename Sto<SS>::T *src, typename Sto<DS>::T *dst, i64 n) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        dst[i] = Sto<DS>::cv(Sto<SS>::ld(src, i));
+}
+
+}  // namespace
+}  // namespace msbg
+
+using namespace msbg;
+
+struct msbg_poisson {
+    Dom dom{};
+    int64_t n_slots = 0, n_active = 0, n_cells = 0;
+    int n_wg = 0;
+    PBlock *d_blocks = nullptr;
+    double2 *d_partial = nullptr;
+    double *d_norm = nullptr;   // [2]
+    float *d_scale = nullptr;   // [2]
+    double *h_norm = nullptr;   // pinned [2]
+    float *d_r = nullptr;       // FP32 residual, n_slots * 4096
+    void *d_e = nullptr, *d_rt = nullptr;  // inner fields (2 or 4 B elements)
+    int inner_bytes = 0;
+    bool e_zeroed = false;
+    // graph of one check period, keyed by its arguments
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int g_mode = -1, g_storage = -1, g_sweeps = -1;
+    float g_omega = 0.0f;
+    const void *g_x = nullptr, *g_b = nullptr;
+};
+
+namespace msbg {
+namespace {
+
+void drop_graph(msbg_poisson *p) {
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+    p->g_mode = -1;
+}
+
+int check_storage(int st) {
+    MSBG_CHECK(st == MSBG_STORAGE_F32 || st == MSBG_STORAGE_F16 || st == MSBG_STORAGE_BF16, MSBG_EINVAL,
+               "storage must be MSBG_STORAGE_F32, _F16 or _BF16");
+    return MSBG_OK;
+}
+
+template <int ST>
+void launch_relax(msbg_poisson *p, hipStream_t s, const void *xin, void *xout, const void *b, float omega, int color,
+                  bool rb) {
+    using T = typename Sto<ST>::T;
+    const float w6 = omega / 6.0f;
+    if (rb)
+        relax_kernel<ST, true><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
+                                                              (const T *)b, w6, color);
+    else
+        relax_kernel<ST, false><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
+                                                               (const T *)b, w6, color);
+}
+
+void relax(msbg_poisson *p, hipStream_t s, int st, const void *xin, void *xout, const void *b, float omega, int color,
+           bool rb) {
+    if (st == MSBG_STORAGE_F32)
+        launch_relax<MSBG_STORAGE_F32>(p, s, xin, xout, b, omega, color, rb);
+    else if (st == MSBG_STORAGE_F16)
+        launch_relax<MSBG_STORAGE_F16>(p, s, xin, xout, b, omega, color, rb);
+    else
+        launch_relax<MSBG_STORAGE_BF16>(p, s, xin, xout, b, omega, color, rb);
+}
+
+void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, float omega, int sweeps) {
+    for (int i = 0; i < sweeps; ++i) {
+        relax(p, s, st, x, x, b, omega, 0, true);
+        relax(p, s, st, x, x, b, omega, 1, true);
+    }
+}
+
+void residual(msbg_poisson *p, hipStream_t s, int st, const void *x, const void *b, float *r) {
+    if (st == MSBG_STORAGE_F32)
+        residual_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)x,
+                                                                        (const float *)b, r, p->d_partial);
+    else if (st == MSBG_STORAGE_F16)
+        residual_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(
+            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
+    else
+        residual_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(
+            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
+    reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, p->n_wg, (double)p->n_cells, p->d_norm, p->d_scale);
+}
+
+void ir_scale(msbg_poisson *p, hipStream_t s, int st) {
+    if (st == MSBG_STORAGE_F32)
+        ir_scale_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
+                                                                        (float *)p->d_rt, (float *)p->d_e);
+    else if (st == MSBG_STORAGE_F16)
+        ir_scale_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
+                                                                        (uint16_t *)p->d_rt, (uint16_t *)p->d_e);
+    else
+        ir_scale_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
+                                                                         (uint16_t *)p->d_rt, (uint16_t *)p->d_e);
+}
+
+void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x) {
+    if (st == MSBG_STORAGE_F32)
+        ir_update_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)p->d_e,
+                                                                         p->d_scale, x);
+    else if (st == MSBG_STORAGE_F16)
+        ir_update_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom,
+                                                                         (const uint16_t *)p->d_e, p->d_scale, x);
+    else
+        ir_update_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom,
+                                                                          (const uint16_t *)p->d_e, p->d_scale, x);
+}
+
+// Inner fields for `storage`; slots outside the listed blocks stay zero (the
+// error equation has homogeneous boundary values).
+int ensure_inner(msbg_poisson *p, int storage) {
+    const int eb = elem_bytes(storage);
+    const size_t n = (size_t)p->n_slots * 4096;
+    if (!p->d_r) MSBG_HIP(hipMalloc(&p->d_r, n * 4));
+    if (p->inner_bytes < eb) {
+        drop_graph(p);
+        if (p->d_e) hipFree(p->d_e);
+        if (p->d_rt) hipFree(p->d_rt);
+        p->d_e = p->d_rt = nullptr;
+        p->inner_bytes = 0;
+        MSBG_HIP(hipMalloc(&p->d_e, n * eb));
+        MSBG_HIP(hipMalloc(&p->d_rt, n * eb));
+        p->inner_bytes = eb;
+        p->e_zeroed = false;
+    }
+    if (!p->e_zeroed) {
+        const i64 words = (i64)(n * p->inner_bytes / 4);
+        zero_u32_kernel<<<1024, 256, 0, stream()>>>((uint32_t *)p->d_e, words);
+        zero_u32_kernel<<<1024, 256, 0, stream()>>>((uint32_t *)p->d_rt, words);
+        MSBG_HIP(hipGetLastError());
+        p->e_zeroed = true;
+    }
+    return MSBG_OK;
+}
+
+// One check period.  mode 0: `sweeps` red-black sweeps on x (FP32), then the
+// residual.  mode 1: refinement step rt = r / s, e = 0, `sweeps` inner sweeps
+// on e in `storage`, x += s e, then the residual (into r).
+void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, const float *b, float omega,
+            int sweeps) {
+    if (mode == 0) {
+        rb_sweeps(p, s, MSBG_STORAGE_F32, x, b, omega, sweeps);
+        residual(p, s, MSBG_STORAGE_F32, x, b, nullptr);
+    } else {
+        ir_scale(p, s, storage);
+        rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps);
+        ir_update(p, s, storage, x);
+        residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
+    }
+}
+
+int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const float *b, float omega, int sweeps) {
+    if (!(p->exec && p->g_mode == mode && p->g_storage == storage && p->g_sweeps == sweeps && p->g_omega == omega &&
+          p->g_x == x && p->g_b == b)) {
+        drop_graph(p);
+        if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
+        period(p, p->cap_stream, mode, storage, x, b, omega, sweeps);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
+        MSBG_HIP(e);
+        e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        MSBG_HIP(e);
+        p->g_mode = mode;
+        p->g_storage = storage;
+        p->g_sweeps = sweeps;
+        p->g_omega = omega;
+        p->g_x = x;
+        p->g_b = b;
+    }
+    MSBG_HIP(hipGraphLaunch(p->exec, stream()));
+    return MSBG_OK;
+}
+
+int read_norms(msbg_poisson *p, double *r2, double *b2) {
+    MSBG_HIP(hipMemcpyAsync(p->h_norm, p->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *r2 = p->h_norm[0];
+    *b2 = p->h_norm[1];
+    return MSBG_OK;
+}
+
+}  // namespace
+}  // namespace msbg
+
+extern "C" {
+
+int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_active, msbg_poisson **out) {
+    MSBG_CHECK(layout && out, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(layout->L == 4, MSBG_EINVAL, "the Poisson band needs 16^3 blocks");
+    MSBG_CHECK(n_active >= 0 && (n_active == 0 || active), MSBG_EINVAL, "bad active list");
+    MSBG_CHECK(n_active <= (int64_t)1 << 28, MSBG_EINVAL, "too many blocks");
+    *out = nullptr;
+    std::vector<PBlock> host((size_t)n_active);
+    std::vector<char> seen((size_t)layout->n_blocks, 0);
+    for (int64_t k = 0; k < n_active; ++k) {
+        const int64_t bid = active[k];
+        MSBG_CHECK(bid >= 0 && bid < layout->n_blocks, MSBG_EINVAL, "block id out of range");
+        MSBG_CHECK(!seen[bid], MSBG_EINVAL, "duplicate block id in the active list");
+        seen[bid] = 1;
+        const int32_t slot = layout->b2s_host[bid];
+        MSBG_CHECK(slot >= 2, MSBG_EINVAL, "listed block is not a value block");
+        const int64_t bx = bid % layout->nx, by = (bid / layout->nx) % layout->ny, bz = bid / layout->nxy;
+        PBlock &B = host[k];
+        B.slot = slot;
+        B.ox = (int32_t)(bx * 16);
+        B.oy = (int32_t)(by * 16);
+        B.oz = (int32_t)(bz * 16);
+        const int64_t c[3] = {bx, by, bz}, n[3] = {layout->nx, layout->ny, layout->nz};
+        for (int f = 0; f < 6; ++f) {
+            const int a = f >> 1, dir = (f & 1) ? 1 : -1;
+            int64_t q[3] = {c[0], c[1], c[2]};
+            q[a] += dir;
+            B.nb[f] = 0;
+            if (q[a] >= 0 && q[a] < n[a]) B.nb[f] = layout->b2s_host[q[0] + q[1] * layout->nx + q[2] * layout->nxy];
+        }
+    }
+    msbg_poisson *p = new msbg_poisson();
+    p->dom = Dom{(int32_t)layout->sx, (int32_t)layout->sy, (int32_t)layout->sz};
+    p->n_slots = layout->n_slots;
+    p->n_active = n_active;
+    int64_t cells = 0;
+    for (const PBlock &B : host) {
+        const int64_t ex = std::min<int64_t>(16, layout->sx - B.ox), ey = std::min<int64_t>(16, layout->sy - B.oy),
+                      ez = std::min<int64_t>(16, layout->sz - B.oz);
+        cells += ex * ey * ez;
+    }
+    p->n_cells = cells;
+    p->n_wg = (int)(n_active * kSlices);
+    hipError_t e = hipSuccess;
+    if (n_active) e = hipMalloc(&p->d_blocks, sizeof(PBlock) * n_active);
+    if (e == hipSuccess && n_active)
+        e = hipMemcpy(p->d_blocks, host.data(), sizeof(PBlock) * n_active, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_partial, sizeof(double2) * std::max(1, p->n_wg));
+    if (e == hipSuccess) e = hipMalloc(&p->d_norm, 2 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&p->d_scale, 2 * sizeof(float));
+    if (e == hipSuccess) e = hipHostMalloc(&p->h_norm, 2 * sizeof(double));
+    if (e != hipSuccess) {
+        msbg_poisson_free(p);
+        return hip_fail(e, "msbg_poisson_new");
+    }
+    *out = p;
+    return MSBG_OK;
+}
+
+int msbg_poisson_free(msbg_poisson *p) {
+    if (!p) return MSBG_OK;
+    drop_graph(p);
+    if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+    hipFree(p->d_blocks);
+    hipFree(p->d_partial);
+    hipFree(p->d_norm);
+    hipFree(p->d_scale);
+    if (p->h_norm) hipHostFree(p->h_norm);
+    hipFree(p->d_r);
+    hipFree(p->d_e);
+    hipFree(p->d_rt);
+    delete p;
+    return MSBG_OK;
+}
+
+int msbg_poisson_info(const msbg_poisson *p, int64_t *n_active, int64_t *n_cells, int64_t *n_slots) {
+    MSBG_CHECK(p && n_active && n_cells && n_slots, MSBG_EINVAL, "null argument");
+    *n_active = p->n_active;
+    *n_cells = p->n_cells;
+    *n_slots = p->n_slots;
+    return MSBG_OK;
+}
+
+int msbg_poisson_jacobi(msbg_poisson *p, int storage, const void *x_in, void *x_out, const void *b, float omega) {
+    MSBG_CHECK(p && x_in && x_out && b, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(x_in != x_out, MSBG_EINVAL, "Jacobi needs distinct input and output fields");
+    int rc = check_storage(storage);
+    if (rc) return rc;
+    if (!p->n_active) return MSBG_OK;
+    relax(p, stream(), storage, x_in, x_out, b, omega, 0, false);
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_rbgs(msbg_poisson *p, int storage, void *x, const void *b, int color, float omega) {
+    MSBG_CHECK(p && x && b, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(color == 0 || color == 1, MSBG_EINVAL, "color must be 0 or 1");
+    int rc = check_storage(storage);
+    if (rc) return rc;
+    if (!p->n_active) return MSBG_OK;
+    relax(p, stream(), storage, x, x, b, omega, color, true);
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_residual(msbg_poisson *p, int storage, const void *x, const void *b, float *r, double *r2,
+                          double *b2) {
+    MSBG_CHECK(p && x && b && r2 && b2, MSBG_EINVAL, "null argument");
+    int rc = check_storage(storage);
+    if (rc) return rc;
+    if (!p->n_active) {
+        *r2 = *b2 = 0.0;
+        return MSBG_OK;
+    }
+    residual(p, stream(), storage, x, b, r);
+    MSBG_HIP(hipGetLastError());
+    return read_norms(p, r2, b2);
+}
+
+int msbg_poisson_convert(int src_storage, const void *src, int dst_storage, void *dst, int64_t n_elems) {
+    MSBG_CHECK(src && dst && n_elems >= 0, MSBG_EINVAL, "bad argument");
+    int rc = check_storage(src_storage);
+    if (rc) return rc;
+    rc = check_storage(dst_storage);
+    if (rc) return rc;
+    if (!n_elems) return MSBG_OK;
+    const int grid = (int)std::min<int64_t>(4096, (n_elems + 255) / 256);
+    hipStream_t s = stream();
+#define MSBG_CONV(A, B)                                                                                          \
+    if (src_storage == A && dst_storage == B)                                                                    \
+        convert_kernel<A, B><<<grid, 256, 0, s>>>((const typename Sto<A>::T *)src, (typename Sto<B>::T *)dst, \
+                                                  n_elems);
+    MSBG_CONV(0, 0) MSBG_CONV(0, 1) MSBG_CONV(0, 2) MSBG_CONV(1, 0) MSBG_CONV(1, 1) MSBG_CONV(1, 2)
+    MSBG_CONV(2, 0) MSBG_CONV(2, 1) MSBG_CONV(2, 2)
+#undef MSBG_CONV
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x, const float *b,
+                       msbg_poisson_stats *st) {
+    MSBG_CHECK(p && prm && x && b && st, MSBG_EINVAL, "null argument");
+    int rc = check_storage(prm->storage);
+    if (rc) return rc;
+    MSBG_CHECK(prm->refine == 0 || prm->refine == 1, MSBG_EINVAL, "refine must be 0 or 1");
+    MSBG_CHECK(prm->refine || prm->storage == MSBG_STORAGE_F32, MSBG_EINVAL,
+               "sweeps on x itself need FP32 storage (use refine = 1 for FP16 / BF16)");
+    MSBG_CHECK(prm->omega > 0.0f && prm->omega < 2.0f, MSBG_EINVAL, "omega must be in (0, 2)");
+    MSBG_CHECK(prm->sweeps_per_check >= 1 && prm->max_sweeps >= 0, MSBG_EINVAL, "bad sweep counts");
+    MSBG_CHECK(prm->tol >= 0.0, MSBG_EINVAL, "negative tolerance");
+    *st = msbg_poisson_stats{};
+    if (!p->n_active) {
+        st->converged = 1;
+        return MSBG_OK;
+    }
+    const int mode = prm->refine;
+    if (mode) {
+        rc = ensure_inner(p, prm->storage);
+        if (rc) return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MSBG_HIP(hipEventCreate(&e0));
+    MSBG_HIP(hipEventCreate(&e1));
+    MSBG_HIP(hipEventRecord(e0, stream()));
+    residual(p, stream(), MSBG_STORAGE_F32, x, b, mode ? p->d_r : nullptr);
+    MSBG_HIP(hipGetLastError());
+    double r2 = 0.0, b2 = 0.0;
+    rc = read_norms(p, &r2, &b2);
+    st->checks = 1;
+    while (rc == MSBG_OK) {
+        const double rel = b2 > 0.0 ? std::sqrt(r2 / b2) : std::sqrt(r2);
+        st->r2 = r2;
+        st->b2 = b2;
+        st->rel_residual = rel;
+        if (rel <= prm->tol) {
+            st->converged = 1;
+            break;
+        }
+        if (!std::isfinite(rel) || st->sweeps + prm->sweeps_per_check > prm->max_sweeps) break;
+        rc = run_period_graph(p, mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+        if (rc) break;
+        st->sweeps += prm->sweeps_per_check;
+        st->checks += 1;
+        rc = read_norms(p, &r2, &b2);
+    }
+    hipError_t e = hipEventRecord(e1, stream());
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    MSBG_HIP(e);
+    st->ms = ms;
+    return MSBG_OK;
+}
+
+}  // extern "C"

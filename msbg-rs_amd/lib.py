@@ -72,8 +72,28 @@ _SIGS = {
     "msbg_particles_to_grid_mac": [vp, vp, vp, vp, vp, i64, f32, C.POINTER(C.c_int64)],
     "msbg_p2g_bspline": [i32, vp, vp, vp, vp, vp, i64, C.POINTER(C.c_int64)],
     "msbg_g2p_bspline": [i32, vp, vp, i64, vp],
+    "msbg_poisson_new": [vp, vp, i64, PP],
+    "msbg_poisson_free": [vp],
+    "msbg_poisson_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+    "msbg_poisson_jacobi": [vp, i32, vp, vp, vp, f32],
+    "msbg_poisson_rbgs": [vp, i32, vp, vp, i32, f32],
+    "msbg_poisson_residual": [vp, i32, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "msbg_poisson_convert": [i32, vp, i32, vp, i64],
+    "msbg_poisson_solve": [vp, vp, vp, vp, vp],
 }
 _RESTYPE = {"msbg_last_error": C.c_char_p}
+
+
+class PoissonParams(C.Structure):
+    """msbg_poisson_params (include/msbg_hip.h)."""
+    _fields_ = [("storage", C.c_int), ("refine", C.c_int), ("omega", C.c_float), ("sweeps_per_check", C.c_int),
+                ("max_sweeps", C.c_int), ("reserved", C.c_int), ("tol", C.c_double)]
+
+
+class PoissonStats(C.Structure):
+    """msbg_poisson_stats (include/msbg_hip.h)."""
+    _fields_ = [("sweeps", C.c_int), ("checks", C.c_int), ("converged", C.c_int), ("reserved", C.c_int),
+                ("rel_residual", C.c_double), ("r2", C.c_double), ("b2", C.c_double), ("ms", C.c_float)]
 
 
 class MsbgError(RuntimeError):

@@ -206,3 +206,14 @@ def reference_shell_blocks(nx, ny, nz):
     d = np.sqrt((BX + 0.5 - cx) ** 2 + (BY + 0.5 - cy) ** 2 + (BZ + 0.5 - cz) ** 2).ravel()
     sel = (d >= r_in) & (d <= r_out)
     return np.nonzero(sel)[0].astype(np.int64)
+
+
+def poisson_rhs(grid: SparseGrid, active, seed=11):
+    """Right-hand side slab (n_slots x 4096, float32) of the band Poisson
+    problems: LCG-uniform values in [-1, 1] on the cells of the listed blocks
+    (block k of `active` takes draws k*4096 .. k*4096+4095), zero elsewhere."""
+    active = np.asarray(active, np.int64)
+    b = np.zeros((grid.n_slots, grid.N), np.float32)
+    u = lcg_uniform(seed, active.size * grid.N).reshape(active.size, grid.N)
+    b[grid.b2s[active]] = (u * np.float32(2.0) - np.float32(1.0)).astype(np.float32)
+    return b

@@ -59,6 +59,21 @@ def load():
     _sig(lib, "orc_bspline_weight", f32, i32, f32, i64)
     _sig(lib, "orc_p2g_bspline", None, i32, i64, i64, i64, f32p, f32p, C.c_void_p, i64, i32p, i64, i64p, i64, f32p)
     _sig(lib, "orc_g2p_bspline", None, i32, i64, i64, i64, f32p, i64, i32p, f32p, i64, f32p)
+    u16 = C.c_uint16
+    _sig(lib, "orc_f32_to_f16", u16, f32)
+    _sig(lib, "orc_f16_to_f32", f32, u16)
+    _sig(lib, "orc_f32_to_bf16", u16, f32)
+    _sig(lib, "orc_bf16_to_f32", f32, u16)
+    vp = C.c_void_p
+    _sig(lib, "orc_poisson_residual", None, i64, i64, i64, i32, vp, vp, i32p, i64p, i64, vp,
+         C.POINTER(C.c_double), C.POINTER(C.c_double))
+    _sig(lib, "orc_jacobi", None, i64, i64, i64, i32, vp, vp, vp, i32p, i64p, i64, f32)
+    _sig(lib, "orc_rbgs", None, i64, i64, i64, i32, vp, vp, i32p, i64p, i64, i32, f32)
+    _sig(lib, "orc_convert", None, i32, vp, i32, vp, i64)
+    _sig(lib, "orc_poisson_solve", None, i64, i64, i64, f32p, f32p, i64, i32p, i64p, i64, i32, i32, f32, i32, i32,
+         C.c_double, np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"))
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    _sig(lib, "orc_cg64", i32, i64, i64, i64, f64p, f64p, i64, i32p, i64p, i64, C.c_double, i32)
     _LIB = Oracle(lib)
     return _LIB
 
