@@ -240,6 +240,7 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
  * Storage types: FP32, IEEE binary16, bfloat16; arithmetic is FP32. */
 enum { MSBG_STORAGE_F32 = 0, MSBG_STORAGE_F16 = 1, MSBG_STORAGE_BF16 = 2 };
 typedef struct msbg_poisson msbg_poisson;
+typedef struct msbg_learned msbg_learned;
 typedef struct {
     int storage;          /* MSBG_STORAGE_*: storage the sweeps read and write */
     int refine;           /* 0: red-black sweeps on x itself (storage must be F32);
@@ -251,11 +252,15 @@ typedef struct {
     int max_sweeps;
     int reserved;         /* 0 */
     double tol;           /* stop when ||b - A x||_2 <= tol * ||b||_2 (FP64 norms) */
+    msbg_learned *learned;   /* NULL, or a network proposing a correction of x after every check period */
+    const msbg_grid *phi;    /* phase field (network input; layout dimensions, 16^3 blocks) */
 } msbg_poisson_params;
 typedef struct {
     int sweeps;           /* red-black sweeps done */
     int checks;           /* residual evaluations */
     int converged;        /* 1 when the tolerance was met */
+    int corrections;      /* learned corrections tried */
+    int accepted;         /* ... and accepted (the true residual decreased) */
     int reserved;
     double rel_residual;  /* ||b - A x|| / ||b|| at the last check */
     double r2, b2;        /* squared norms at the last check */
@@ -282,6 +287,47 @@ int msbg_poisson_convert(int src_storage, const void *src, int dst_storage, void
  * check period runs as one replayed HIP graph. */
 int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *params, float *x, const float *b,
                        msbg_poisson_stats *stats);
+
+/* ---- learned error correction ---------------------------------------------
+ * No reference counterpart (the reference has no learned component).  A
+ * 4-layer 3^3 convolution network per 16^3 block: input X0 = 20^3 x 2 bf16
+ * (the block with a 2-cell halo; channel 0 = residual / 2^k, channel 1 =
+ * phase field), L1 valid 2->16, L2 valid 16->16, L3 same 16->16 (ReLU after
+ * each), L4 same 16->1, output Y = 16^3 f32 (the error estimate / 2^k /
+ * out_scale).  Tensors are channels-last [z][y][x][c].  Parameters (FP32):
+ * 4 layers x (W[27][16][16] + bias[16]), tap t = dx + 3 dy + 9 dz; layer 1
+ * uses input channels 0..1, layer 4 output channel 0, the rest is zero.
+ * Forward and backward run as BF16 MFMA implicit GEMMs with FP32 accumulation. */
+int msbg_learned_param_count(int *n);
+int msbg_learned_new(msbg_learned **out); /* parameters zero */
+int msbg_learned_free(msbg_learned *net);
+int msbg_learned_set_params(msbg_learned *net, const float *host_params, float out_scale); /* resets Adam */
+int msbg_learned_get_params(const msbg_learned *net, float *host_params, float *out_scale);
+/* d_y[t] (16^3 f32) = network output for d_x0[t] (20^3 x 2 bf16), t < n. */
+int msbg_learned_forward(msbg_learned *net, const uint16_t *d_x0, int64_t n, float *d_y);
+/* Loss sum (y - target)^2 / (2 n 4096) over n tiles and its gradient with
+ * respect to the parameters (host_grad: msbg_learned_param_count floats). */
+int msbg_learned_loss_grad(msbg_learned *net, const uint16_t *d_x0, const float *d_target, int64_t n,
+                           double *loss, float *host_grad);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * pairs drawn uniformly (SplitMix64 from `seed`) from n_pairs (x0, target)
+ * pairs; host_losses[step] = the batch loss before the step. */
+int msbg_learned_train(msbg_learned *net, const uint16_t *d_x0, const float *d_target, int64_t n_pairs, int steps,
+                       int batch, float lr, uint64_t seed, float *host_losses);
+/* Network input of every listed block of a Poisson band (d_x0: n_active x
+ * 20^3 x 2): channel 0 = d_r * inv_scale (0 in unlisted blocks), channel 1 =
+ * the phase field; halo cells outside the domain mirror. */
+int msbg_poisson_net_input(msbg_poisson *p, const msbg_grid *phi, const float *d_r, float inv_scale,
+                           uint16_t *d_x0);
+/* Training target of every listed block: (x* - x) * inv_scale (n_active x 16^3). */
+int msbg_poisson_error_target(msbg_poisson *p, const float *d_x, const float *d_xstar, float inv_scale,
+                              float *d_target);
+/* One learned correction of x: r = b - A x, 2^k ~ rms(r), delta = 2^k
+ * out_scale Y(net input), x' = x + alpha delta with alpha = (r . A delta) /
+ * |A delta|^2 (FP64), kept only if ||b - A x'|| < ||b - A x|| (the true
+ * residual decides; otherwise x is unchanged). */
+int msbg_poisson_learned_correct(msbg_poisson *p, msbg_learned *net, const msbg_grid *phi, float *x, const float *b,
+                                 double *r2_before, double *r2_after, int *accepted);
 
 #ifdef __cplusplus
 }

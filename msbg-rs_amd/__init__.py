@@ -14,3 +14,4 @@ from .device import (  # noqa: F401
     downsample_channel_avg, halo_fill, halo_fill_multires, halo_topology, regularize_refinement_map, sweep_multires,
 )
 from .solve import PoissonBand, Storage  # noqa: F401
+from .learned import CorrectorNet  # noqa: F401

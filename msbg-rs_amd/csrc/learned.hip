@@ -1,0 +1,746 @@
+// Learned error correction for the Poisson band: a 4-layer 3^3 convolution
+// network on 16^3 blocks with a 2-cell halo, BF16 operands on the MFMA
+// matrix cores with FP32 accumulation, trained on the GPU by the kernels
+// below (forward, backward-data, backward-weights, Adam).  No reference
+// counterpart: the reference has no learned component.
+//
+// Network (per block): X0 = 20^3 x 2 (scaled residual, phase field),
+//   L1 valid 20^3 x 2 -> 18^3 x 16, ReLU     L2 valid 18^3 -> 16^3 x 16, ReLU
+//   L3 same 16^3 -> 16^3 x 16, ReLU          L4 same 16^3 -> 16^3 x 1
+// out[p][o] = bias[o] + sum_t sum_c W[t][c][o] in[p + off + d_t][c], tap
+// t = dx + 3 dy + 9 dz, off = 0 (valid) or -1 (same, zero padding).
+// Parameters (FP32 master copy): per layer W[27][16][16] + bias[16], padded
+// (layer 1 uses c < 2, layer 4 uses o < 1; the padding stays zero because
+// its gradient is zero).  Tensors are channels-last [tile][z][y][x][c], BF16.
+//
+// Every convolution -- forward, and the data gradient (the same kernel with
+// the flipped, channel-transposed weights and offset -off-2) -- is an implicit
+// GEMM: a wave computes 16 output cells x 16 channels with 27 MFMA
+// 16x16x16 steps (one per tap, K = 16 input channels).  The weight gradient
+// is a second GEMM with K running over cells.
+#include <cmath>
+#include <vector>
+
+#include "poisson.hpp"
+
+namespace msbg {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLP = 27 * 16 * 16 + 16;  // parameters per layer (padded)
+constexpr int kParams = 4 * kLP;
+constexpr int kTap = 27 * 16 * 16;
+
+__device__ __forceinline__ uint16_t to_bf16(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+struct ConvArgs {
+    const uint16_t *in;
+    const int32_t *idx;  // tile index into `in` per batch position, or null
+    int sin, sout, off;
+    const uint16_t *w;   // [27][16][16] bf16
+    const float *bias;   // [16] or null
+    int relu;
+    const uint16_t *mask;  // [tile][sout^3][16]: output zeroed where mask <= 0, or null
+    void *out;             // bf16 [tile][sout^3][16], or f32 [tile][sout^3] (channel 0)
+    int groups;            // ceil(sout^3 / 16)
+    int n_tiles;
+};
+
+// MFMA 16x16x16 bf16 lane layout: A[m][k] from lane m + 16 (k / 4), B[k][n]
+// from lane n + 16 (k / 4), D[m][n] in lane n + 16 (m / 4), element m % 4.
+template <int CIN, bool OUTF>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = wave / a.groups, grp = wave - tile * a.groups;
+    if (tile >= a.n_tiles) return;  // wave-uniform
+    const int ncell = a.sout * a.sout * a.sout;
+    const int m = lane & 15, kg = lane >> 4;
+    int q = grp * 16 + m;
+    if (q >= ncell) q = ncell - 1;
+    const int qx = q % a.sout, qy = (q / a.sout) % a.sout, qz = q / (a.sout * a.sout);
+    const int64_t in_base = (int64_t)(a.idx ? a.idx[tile] : tile) * a.sin * a.sin * a.sin;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < 27; ++t) {
+        const uint16_t *wp = a.w + (t * 16 + 4 * kg) * 16 + m;
+        const s4 bv = {(short)wp[0], (short)wp[16], (short)wp[32], (short)wp[48]};
+        const int ix = qx + a.off + t % 3, iy = qy + a.off + (t / 3) % 3, iz = qz + a.off + t / 9;
+        s4 av = {0, 0, 0, 0};
+        if ((unsigned)ix < (unsigned)a.sin && (unsigned)iy < (unsigned)a.sin && (unsigned)iz < (unsigned)a.sin) {
+            const int64_t cell = in_base + ((int64_t)iz * a.sin + iy) * a.sin + ix;
+            if (CIN == 16) {
+                const uint2 v = *(const uint2 *)(a.in + cell * 16 + 4 * kg);
+                av = s4{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+            } else if (kg == 0) {
+                const uint32_t v = *(const uint32_t *)(a.in + cell * 2);
+                av = s4{(short)(v & 0xffff), (short)(v >> 16), 0, 0};
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    const float bias = a.bias ? a.bias[m] : 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        const int cell = grp * 16 + 4 * kg + j;
+        if (cell >= ncell) continue;
+        float v = acc[j] + bias;
+        if (a.relu) v = v > 0.0f ? v : 0.0f;
+        const int64_t o = (int64_t)tile * ncell + cell;
+        if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
+        if (OUTF) {
+            if (m == 0) ((float *)a.out)[o] = v;
+        } else {
+            ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
+        }
+    }
+}
+
+struct WgArgs {
+    const uint16_t *X;
+    const int32_t *idx;
+    int sin, sout, off;
+    const uint16_t *G;  // [tile][sout^3][16]
+    int chunks;         // waves per tile
+    int gpc;            // 16-cell groups per wave
+    int n_tiles;
+    float *part;        // [wave][kLP]
+};
+
+// Weight / bias gradient partials of one layer: wave (tile, chunk) sums over
+// its cells dW[t][c][o] += X[p + off + d_t][c] G[p][o] (MFMA with M = c,
+// N = o, K = 16 cells) and db[o] += G[p][o].
+template <int CIN>
+__global__ __launch_bounds__(256) void wgrad_kernel(WgArgs a) {
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int tile = wave / a.chunks, chunk = wave - tile * a.chunks;
+    if (tile >= a.n_tiles) return;
+    const int ncell = a.sout * a.sout * a.sout;
+    const int groups = (ncell + 15) / 16;
+    const int m = lane & 15, kg = lane >> 4;
+    const int64_t xbase = (int64_t)(a.idx ? a.idx[tile] : tile) * a.sin * a.sin * a.sin;
+    f4 acc[27];
+    for (int t = 0; t < 27; ++t) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    const int g0 = chunk * a.gpc, g1 = min(groups, g0 + a.gpc);
+    for (int g = g0; g < g1; ++g) {
+        s4 bv;
+        int cx[4], cy[4], cz[4];
+        bool cvalid[4];
+        for (int i = 0; i < 4; ++i) {
+            const int c = g * 16 + 4 * kg + i;
+            cvalid[i] = c < ncell;
+            const int cc = cvalid[i] ? c : 0;
+            cx[i] = cc % a.sout;
+            cy[i] = (cc / a.sout) % a.sout;
+            cz[i] = cc / (a.sout * a.sout);
+            const uint16_t gv = cvalid[i] ? a.G[((int64_t)tile * ncell + c) * 16 + m] : (uint16_t)0;
+            bv[i] = (short)gv;
+            bsum += from_bf16(gv);
+        }
+        for (int t = 0; t < 27; ++t) {
+            s4 av = {0, 0, 0, 0};
+            if (m < CIN) {
+                for (int i = 0; i < 4; ++i) {
+                    const int ix = cx[i] + a.off + t % 3, iy = cy[i] + a.off + (t / 3) % 3,
+                              iz = cz[i] + a.off + t / 9;
+                    if (cvalid[i] && (unsigned)ix < (unsigned)a.sin && (unsigned)iy < (unsigned)a.sin &&
+                        (unsigned)iz < (unsigned)a.sin)
+                        av[i] = (short)a.X[(xbase + ((int64_t)iz * a.sin + iy) * a.sin + ix) * CIN + m];
+                }
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+        }
+    }
+    float *P = a.part + (int64_t)wave * kLP;
+    for (int t = 0; t < 27; ++t)
+        for (int j = 0; j < 4; ++j) P[(t * 16 + 4 * kg + j) * 16 + m] = acc[t][j];
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (kg == 0) P[kTap + m] = bsum;
+}
+
+// grad[k] = sum over the n partials in order (deterministic).
+__global__ void reduce_partials_kernel(const float *__restrict__ part, int n, float *grad) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kLP) return;
+    float s = 0.0f;
+    for (int w = 0; w < n; ++w) s += part[(int64_t)w * kLP + k];
+    grad[k] = s;
+}
+
+// bf16 weights and their flipped, channel-transposed copy for the data gradient:
+// wf[l][26 - t][o][c] = w[l][t][c][o].
+__global__ void prep_weights_kernel(const float *__restrict__ params, uint16_t *w, uint16_t *wf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * kTap) return;
+    const int l = i / kTap, r = i % kTap;
+    const int t = r / 256, c = (r / 16) % 16, o = r % 16;
+    const uint16_t v = to_bf16(params[(int64_t)l * kLP + r]);
+    w[i] = v;
+    wf[l * kTap + ((26 - t) * 16 + o) * 16 + c] = v;
+}
+
+// Loss (y - target)^2 / (2 n 4096): G4 = (y - target) / (n 4096) in channel 0,
+// per-work-group partial sums of (y - target)^2.
+__global__ __launch_bounds__(256) void loss_kernel(const float *__restrict__ y, const float *__restrict__ target,
+                                                   const int32_t *idx, int n, float inv, uint16_t *g4,
+                                                   double *part) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double d2 = 0.0;
+    if (i < (int64_t)n * 4096) {
+        const int b = (int)(i >> 12), c = (int)(i & 4095);
+        const float d = y[i] - target[(int64_t)(idx ? idx[b] : b) * 4096 + c];
+        d2 = (double)d * (double)d;
+        uint16_t *g = g4 + i * 16;
+        g[0] = to_bf16(d * inv);
+        for (int k = 1; k < 16; ++k) g[k] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) d2 += __shfl_xor(d2, o);
+    __shared__ double s[4];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+__global__ void loss_reduce_kernel(const double *__restrict__ part, int n, double scale, double *loss_out,
+                                   float *loss_f) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *loss_out = s * scale;
+    if (loss_f) *loss_f = (float)(s * scale);
+}
+
+__global__ void adam_kernel(float *p, float *m, float *v, const float *__restrict__ g, int n, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---- band <-> tiles -------------------------------------------------------
+__device__ __forceinline__ int mirror(int g, int s) { return g < 0 ? -g - 1 : (g >= s ? 2 * s - 1 - g : g); }
+
+// X0[k] (20^3 x 2, bf16) of listed block k: channel 0 = r * inv_scale (0 in
+// blocks that are not listed), channel 1 = phi; halo cells outside the domain
+// mirror (Neumann).  inv_scale = scale[1] if scale is given, else inv_host.
+__global__ __launch_bounds__(256) void net_input_kernel(const PBlock *__restrict__ blocks, Dom d, int nx, int ny,
+                                                        const int32_t *__restrict__ b2s,
+                                                        const uint8_t *__restrict__ listed, const float *r,
+                                                        const float *scale, float inv_host, const float *phi_slab,
+                                                        const int32_t *__restrict__ phi_b2s, uint16_t *x0, int n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * 8000) return;
+    const int k = (int)(i / 8000), c = (int)(i % 8000);
+    const PBlock B = blocks[k];
+    const int gx = mirror(B.ox + c % 20 - 2, d.sx), gy = mirror(B.oy + (c / 20) % 20 - 2, d.sy),
+              gz = mirror(B.oz + c / 400 - 2, d.sz);
+    const int64_t bid = (gx >> 4) + (int64_t)(gy >> 4) * nx + (int64_t)(gz >> 4) * nx * ny;
+    const int vid = (gx & 15) | (gy & 15) << 4 | (gz & 15) << 8;
+    const float inv = scale ? scale[1] : inv_host;
+    const float rv = listed[bid] ? r[((int64_t)b2s[bid] << 12) + vid] * inv : 0.0f;
+    const float pv = phi_slab[((int64_t)phi_b2s[bid] << 12) + vid];
+    x0[i * 2] = to_bf16(rv);
+    x0[i * 2 + 1] = to_bf16(pv);
+}
+
+// target[k][vid] = (xs - x) * inv for the in-domain cells of listed block k, else 0.
+__global__ __launch_bounds__(256) void error_target_kernel(const PBlock *__restrict__ blocks, Dom d, const float *x,
+                                                           const float *xs, float inv, float *target, int n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * 4096) return;
+    const int k = (int)(i >> 12), v = (int)(i & 4095);
+    const PBlock B = blocks[k];
+    const bool in = B.ox + (v & 15) < d.sx && B.oy + ((v >> 4) & 15) < d.sy && B.oz + (v >> 8) < d.sz;
+    const int64_t c = ((int64_t)B.slot << 12) + v;
+    target[i] = in ? (xs[c] - x[c]) * inv : 0.0f;
+}
+
+// delta[slot][vid] = y[k][vid] * out_scale * scale[0] on in-domain cells of listed blocks.
+__global__ __launch_bounds__(256) void delta_kernel(const PBlock *__restrict__ blocks, Dom d, const float *y,
+                                                    float out_scale, const float *scale, float *delta, int n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * 4096) return;
+    const int k = (int)(i >> 12), v = (int)(i & 4095);
+    const PBlock B = blocks[k];
+    const bool in = B.ox + (v & 15) < d.sx && B.oy + ((v >> 4) & 15) < d.sy && B.oz + (v >> 8) < d.sz;
+    if (in) delta[((int64_t)B.slot << 12) + v] = (y[i] * out_scale) * scale[0];
+}
+
+// q = A delta (delta is zero outside the listed cells); partials of r . q and q . q.
+__global__ __launch_bounds__(kThreads) void dots_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                        const float *delta, const float *r, double2 *partial) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    double rq = 0.0, qq = 0.0;
+    if (gx < d.sx && gy < d.sy) {
+        const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+        for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+            const int gz = B.oz + tz;
+            if (gz >= d.sz) break;
+            const int vid = tx | ty << 4 | tz << 8;
+            const int64_t c = ((int64_t)B.slot << 12) + vid;
+            const float xl = tx > 0 ? delta[c - 1] : (gx == 0 ? delta[c] : delta[((int64_t)B.nb[0] << 12) + (vid | 15)]);
+            const float xr = gx + 1 >= d.sx ? delta[c]
+                                            : (tx < 15 ? delta[c + 1] : delta[((int64_t)B.nb[1] << 12) + (vid & ~15)]);
+            const float yd = ty > 0 ? delta[c - 16]
+                                    : (gy == 0 ? delta[c] : delta[((int64_t)B.nb[2] << 12) + (vid | 0xf0)]);
+            const float yu = gy + 1 >= d.sy
+                                 ? delta[c]
+                                 : (ty < 15 ? delta[c + 16] : delta[((int64_t)B.nb[3] << 12) + (vid & ~0xf0)]);
+            const float zb = tz > 0 ? delta[c - 256]
+                                    : (gz == 0 ? delta[c] : delta[((int64_t)B.nb[4] << 12) + (vid | 0xf00)]);
+            const float zf = gz + 1 >= d.sz
+                                 ? delta[c]
+                                 : (tz < 15 ? delta[c + 256] : delta[((int64_t)B.nb[5] << 12) + (vid & ~0xf00)]);
+            const float s = ((((xl + xr) + yd) + yu) + zb) + zf;
+            const float q = __builtin_fmaf(6.0f, delta[c], -s);
+            rq += (double)r[c] * (double)q;
+            qq += (double)q * (double)q;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rq += __shfl_xor(rq, o);
+        qq += __shfl_xor(qq, o);
+    }
+    __shared__ double s_a[kThreads / 64], s_b[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s_a[threadIdx.x >> 6] = rq;
+        s_b[threadIdx.x >> 6] = qq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+            a += s_a[w];
+            b += s_b[w];
+        }
+        partial[blockIdx.x] = make_double2(a, b);
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_dots_kernel(const double2 *__restrict__ partial, int n, double *out) {
+    __shared__ double s_a[256], s_b[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partial[i].x;
+        b += partial[i].y;
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_a[threadIdx.x] += s_a[threadIdx.x + o];
+            s_b[threadIdx.x] += s_b[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_a[0];
+        out[1] = s_b[0];
+    }
+}
+
+// x_try = fmaf(alpha, delta, x) on the listed in-domain cells, alpha = (r.q)/(q.q)
+// (0 when q = 0); commit = 1 copies x_try back into x instead.
+__global__ __launch_bounds__(kThreads) void try_kernel(const PBlock *__restrict__ blocks, Dom d, const double *dots,
+                                                       const float *delta, float *x, float *xt, int commit) {
+    const PBlock B = blocks[blockIdx.x / kSlices];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = B.ox + tx, gy = B.oy + ty;
+    if (gx >= d.sx || gy >= d.sy) return;
+    const float alpha = dots[1] > 0.0 ? (float)(dots[0] / dots[1]) : 0.0f;
+    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
+        if (B.oz + tz >= d.sz) break;
+        const int64_t c = ((int64_t)B.slot << 12) + (tx | ty << 4 | tz << 8);
+        if (commit)
+            x[c] = xt[c];
+        else
+            xt[c] = __builtin_fmaf(alpha, delta[c], x[c]);
+    }
+}
+
+__global__ void fill_zero_kernel(float *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+int blocks_for(int64_t threads) { return (int)((threads + 255) / 256); }
+
+}  // namespace
+}  // namespace msbg
+
+using namespace msbg;
+
+struct msbg_learned {
+    float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr;
+    uint16_t *d_w = nullptr, *d_wf = nullptr;
+    float out_scale = 1.0f;
+    int adam_t = 0;
+    // workspace for `cap` tiles
+    int64_t cap = 0;
+    uint16_t *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *G1 = nullptr, *G2 = nullptr, *G3 = nullptr, *G4 = nullptr;
+    uint16_t *X0 = nullptr;  // network input assembled from a Poisson band
+    float *Y = nullptr;
+    float *wpart = nullptr;
+    double *loss_part = nullptr, *d_loss = nullptr;
+    int64_t cap_train = 0;
+};
+
+namespace msbg {
+namespace {
+
+void free_ws(msbg_learned *n) {
+    for (uint16_t **p : {&n->A1, &n->A2, &n->A3, &n->G1, &n->G2, &n->G3, &n->G4, &n->X0}) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    hipFree(n->Y);
+    hipFree(n->wpart);
+    hipFree(n->loss_part);
+    n->Y = nullptr;
+    n->wpart = nullptr;
+    n->loss_part = nullptr;
+    n->cap = n->cap_train = 0;
+}
+
+// Workspace for `tiles` tiles; `train` adds the backward tensors.
+int ensure_ws(msbg_learned *n, int64_t tiles, bool train) {
+    if (tiles <= n->cap && (!train || tiles <= n->cap_train)) return MSBG_OK;
+    free_ws(n);
+    const size_t a18 = (size_t)tiles * 5832 * 16 * 2, a16 = (size_t)tiles * 4096 * 16 * 2;
+    MSBG_HIP(hipMalloc(&n->A1, a18));
+    MSBG_HIP(hipMalloc(&n->A2, a16));
+    MSBG_HIP(hipMalloc(&n->A3, a16));
+    MSBG_HIP(hipMalloc(&n->Y, (size_t)tiles * 4096 * 4));
+    MSBG_HIP(hipMalloc(&n->X0, (size_t)tiles * 8000 * 2 * 2));
+    n->cap = tiles;
+    if (train) {
+        MSBG_HIP(hipMalloc(&n->G1, a18));
+        MSBG_HIP(hipMalloc(&n->G2, a16));
+        MSBG_HIP(hipMalloc(&n->G3, a16));
+        MSBG_HIP(hipMalloc(&n->G4, a16));
+        MSBG_HIP(hipMalloc(&n->wpart, (size_t)tiles * 23 * kLP * 4));
+        MSBG_HIP(hipMalloc(&n->loss_part, (size_t)tiles * 16 * sizeof(double)));
+        n->cap_train = tiles;
+    }
+    return MSBG_OK;
+}
+
+template <int CIN, bool OUTF>
+void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int sout, int off, const uint16_t *w,
+          const float *bias, int relu, const uint16_t *mask, void *out, int n) {
+    ConvArgs a;
+    a.in = in;
+    a.idx = idx;
+    a.sin = sin;
+    a.sout = sout;
+    a.off = off;
+    a.w = w;
+    a.bias = bias;
+    a.relu = relu;
+    a.mask = mask;
+    a.out = out;
+    a.groups = (sout * sout * sout + 15) / 16;
+    a.n_tiles = n;
+    const int64_t waves = (int64_t)a.groups * n;
+    conv_kernel<CIN, OUTF><<<(int)((waves + 3) / 4), 256, 0, s>>>(a);
+}
+
+void forward(msbg_learned *net, hipStream_t s, const uint16_t *x0, const int32_t *idx, int n) {
+    const uint16_t *W = net->d_w;
+    const float *P = net->d_params;
+    conv<2, false>(s, x0, idx, 20, 18, 0, W, P + 0 * kLP + kTap, 1, nullptr, net->A1, n);
+    conv<16, false>(s, net->A1, nullptr, 18, 16, 0, W + kTap, P + 1 * kLP + kTap, 1, nullptr, net->A2, n);
+    conv<16, false>(s, net->A2, nullptr, 16, 16, -1, W + 2 * kTap, P + 2 * kLP + kTap, 1, nullptr, net->A3, n);
+    conv<16, true>(s, net->A3, nullptr, 16, 16, -1, W + 3 * kTap, P + 3 * kLP + kTap, 0, nullptr, net->Y, n);
+}
+
+template <int CIN>
+void wgrad(msbg_learned *net, hipStream_t s, const uint16_t *X, const int32_t *idx, int sin, int sout, int off,
+           const uint16_t *G, int n, float *grad) {
+    WgArgs a;
+    a.X = X;
+    a.idx = idx;
+    a.sin = sin;
+    a.sout = sout;
+    a.off = off;
+    a.G = G;
+    a.gpc = 16;
+    a.chunks = ((sout * sout * sout + 15) / 16 + a.gpc - 1) / a.gpc;
+    a.n_tiles = n;
+    a.part = net->wpart;
+    const int64_t waves = (int64_t)a.chunks * n;
+    wgrad_kernel<CIN><<<(int)((waves + 3) / 4), 256, 0, s>>>(a);
+    reduce_partials_kernel<<<(kLP + 255) / 256, 256, 0, s>>>(net->wpart, (int)waves, grad);
+}
+
+// Loss and gradients of n tiles into net->d_grad; loss into net->d_loss (and loss_f if given).
+void loss_grad(msbg_learned *net, hipStream_t s, const uint16_t *x0, const float *target, const int32_t *idx, int n,
+               float *loss_f) {
+    forward(net, s, x0, idx, n);
+    const double inv = 1.0 / ((double)n * 4096.0);
+    const int nb = n * 16;
+    loss_kernel<<<nb, 256, 0, s>>>(net->Y, target, idx, n, (float)inv, net->G4, net->loss_part);
+    loss_reduce_kernel<<<1, 64, 0, s>>>(net->loss_part, nb, 0.5 * inv, net->d_loss, loss_f);
+    const uint16_t *Wf = net->d_wf;
+    float *g = net->d_grad;
+    wgrad<16>(net, s, net->A3, nullptr, 16, 16, -1, net->G4, n, g + 3 * kLP);
+    conv<16, false>(s, net->G4, nullptr, 16, 16, -1, Wf + 3 * kTap, nullptr, 0, net->A3, net->G3, n);
+    wgrad<16>(net, s, net->A2, nullptr, 16, 16, -1, net->G3, n, g + 2 * kLP);
+    conv<16, false>(s, net->G3, nullptr, 16, 16, -1, Wf + 2 * kTap, nullptr, 0, net->A2, net->G2, n);
+    wgrad<16>(net, s, net->A1, nullptr, 18, 16, 0, net->G2, n, g + 1 * kLP);
+    conv<16, false>(s, net->G2, nullptr, 16, 18, -2, Wf + 1 * kTap, nullptr, 0, net->A1, net->G1, n);
+    wgrad<2>(net, s, x0, idx, 20, 18, 0, net->G1, n, g + 0 * kLP);
+}
+
+void prep(msbg_learned *net, hipStream_t s) {
+    prep_weights_kernel<<<blocks_for(4 * kTap), 256, 0, s>>>(net->d_params, net->d_w, net->d_wf);
+}
+
+}  // namespace
+
+namespace learned {
+
+int correct(msbg_poisson *p, msbg_learned *net, const msbg_grid *phi, float *x, const float *b, double r2_in,
+            double *r2, double *b2, int *accepted) {
+    hipStream_t s = stream();
+    const size_t slab = (size_t)p->n_slots * 4096;
+    if (!p->d_delta) {
+        MSBG_HIP(hipMalloc(&p->d_delta, slab * 4));
+        MSBG_HIP(hipMalloc(&p->d_xtry, slab * 4));
+        MSBG_HIP(hipMalloc(&p->d_dots, 2 * sizeof(double)));
+        fill_zero_kernel<<<1024, 256, 0, s>>>(p->d_delta, (int64_t)slab);
+    }
+    const int64_t chunk = std::min<int64_t>(p->n_active, 512);
+    int rc = ensure_ws(net, chunk, false);
+    if (rc) return rc;
+    uint16_t *x0 = net->X0;
+    for (int64_t k0 = 0; k0 < p->n_active; k0 += chunk) {
+        const int n = (int)std::min<int64_t>(chunk, p->n_active - k0);
+        net_input_kernel<<<blocks_for((int64_t)n * 8000), 256, 0, s>>>(
+            p->d_blocks + k0, p->dom, (int)p->nx, (int)p->ny, p->d_b2s, p->d_listed, p->d_r, p->d_scale, 0.0f,
+            phi->d_slab, phi->d_b2s, x0, n);
+        forward(net, s, x0, nullptr, n);
+        delta_kernel<<<blocks_for((int64_t)n * 4096), 256, 0, s>>>(p->d_blocks + k0, p->dom, net->Y, net->out_scale,
+                                                                   p->d_scale, p->d_delta, n);
+    }
+    dots_kernel<<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_delta, p->d_r, p->d_partial);
+    reduce_dots_kernel<<<1, 256, 0, s>>>(p->d_partial, p->n_wg, p->d_dots);
+    MSBG_HIP(hipMemcpyAsync(p->d_xtry, x, slab * 4, hipMemcpyDeviceToDevice, s));
+    try_kernel<<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_dots, p->d_delta, x, p->d_xtry, 0);
+    pband::residual(p, s, MSBG_STORAGE_F32, p->d_xtry, b, p->d_r);
+    MSBG_HIP(hipGetLastError());
+    rc = pband::read_norms(p, r2, b2);
+    if (rc) return rc;
+    *accepted = *r2 < r2_in;
+    if (*accepted) {
+        try_kernel<<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_dots, p->d_delta, x, p->d_xtry, 1);
+    } else {
+        pband::residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
+        MSBG_HIP(hipGetLastError());
+        rc = pband::read_norms(p, r2, b2);
+        if (rc) return rc;
+    }
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+}  // namespace learned
+}  // namespace msbg
+
+extern "C" {
+
+int msbg_learned_param_count(int *n) {
+    MSBG_CHECK(n, MSBG_EINVAL, "null argument");
+    *n = kParams;
+    return MSBG_OK;
+}
+
+int msbg_learned_new(msbg_learned **out) {
+    MSBG_CHECK(out, MSBG_EINVAL, "null argument");
+    msbg_learned *n = new msbg_learned();
+    hipError_t e = hipMalloc(&n->d_params, kParams * 4);
+    if (e == hipSuccess) e = hipMalloc(&n->d_m, kParams * 4);
+    if (e == hipSuccess) e = hipMalloc(&n->d_v, kParams * 4);
+    if (e == hipSuccess) e = hipMalloc(&n->d_grad, kParams * 4);
+    if (e == hipSuccess) e = hipMalloc(&n->d_w, 4 * kTap * 2);
+    if (e == hipSuccess) e = hipMalloc(&n->d_wf, 4 * kTap * 2);
+    if (e == hipSuccess) e = hipMalloc(&n->d_loss, sizeof(double));
+    if (e == hipSuccess) e = hipMemset(n->d_params, 0, kParams * 4);
+    if (e == hipSuccess) e = hipMemset(n->d_m, 0, kParams * 4);
+    if (e == hipSuccess) e = hipMemset(n->d_v, 0, kParams * 4);
+    if (e != hipSuccess) {
+        msbg_learned_free(n);
+        return hip_fail(e, "msbg_learned_new");
+    }
+    prep(n, stream());
+    *out = n;
+    return MSBG_OK;
+}
+
+int msbg_learned_free(msbg_learned *n) {
+    if (!n) return MSBG_OK;
+    free_ws(n);
+    hipFree(n->d_params);
+    hipFree(n->d_m);
+    hipFree(n->d_v);
+    hipFree(n->d_grad);
+    hipFree(n->d_w);
+    hipFree(n->d_wf);
+    hipFree(n->d_loss);
+    delete n;
+    return MSBG_OK;
+}
+
+int msbg_learned_set_params(msbg_learned *n, const float *host_params, float out_scale) {
+    MSBG_CHECK(n && host_params, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(std::isfinite(out_scale), MSBG_EINVAL, "out_scale must be finite");
+    MSBG_HIP(hipMemcpyAsync(n->d_params, host_params, kParams * 4, hipMemcpyHostToDevice, stream()));
+    MSBG_HIP(hipMemsetAsync(n->d_m, 0, kParams * 4, stream()));
+    MSBG_HIP(hipMemsetAsync(n->d_v, 0, kParams * 4, stream()));
+    n->adam_t = 0;
+    n->out_scale = out_scale;
+    prep(n, stream());
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_learned_get_params(const msbg_learned *n, float *host_params, float *out_scale) {
+    MSBG_CHECK(n && host_params && out_scale, MSBG_EINVAL, "null argument");
+    MSBG_HIP(hipMemcpyAsync(host_params, n->d_params, kParams * 4, hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *out_scale = n->out_scale;
+    return MSBG_OK;
+}
+
+int msbg_learned_forward(msbg_learned *n, const uint16_t *d_x0, int64_t count, float *d_y) {
+    MSBG_CHECK(n && d_x0 && d_y && count >= 0, MSBG_EINVAL, "bad argument");
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(count, 1), 512);
+    int rc = ensure_ws(n, chunk, false);
+    if (rc) return rc;
+    for (int64_t k0 = 0; k0 < count; k0 += chunk) {
+        const int c = (int)std::min<int64_t>(chunk, count - k0);
+        forward(n, stream(), d_x0 + k0 * 16000, nullptr, c);
+        MSBG_HIP(hipMemcpyAsync(d_y + k0 * 4096, n->Y, (size_t)c * 4096 * 4, hipMemcpyDeviceToDevice, stream()));
+    }
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_learned_loss_grad(msbg_learned *n, const uint16_t *d_x0, const float *d_target, int64_t count,
+                           double *loss, float *host_grad) {
+    MSBG_CHECK(n && d_x0 && d_target && loss && host_grad, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(count >= 1 && count <= 4096, MSBG_EINVAL, "1 <= count <= 4096 tiles");
+    int rc = ensure_ws(n, count, true);
+    if (rc) return rc;
+    loss_grad(n, stream(), d_x0, d_target, nullptr, (int)count, nullptr);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipMemcpyAsync(host_grad, n->d_grad, kParams * 4, hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipMemcpyAsync(loss, n->d_loss, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_learned_train(msbg_learned *n, const uint16_t *d_x0, const float *d_target, int64_t n_pairs, int steps,
+                       int batch, float lr, uint64_t seed, float *host_losses) {
+    MSBG_CHECK(n && d_x0 && d_target && host_losses, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(n_pairs >= 1 && n_pairs < ((int64_t)1 << 31), MSBG_EINVAL, "bad pair count");
+    MSBG_CHECK(steps >= 0 && batch >= 1 && batch <= 1024, MSBG_EINVAL, "bad steps / batch");
+    if (!steps) return MSBG_OK;
+    int rc = ensure_ws(n, batch, true);
+    if (rc) return rc;
+    hipStream_t s = stream();
+    // batch indices: SplitMix64 stream from `seed`, uniform over the pairs
+    std::vector<int32_t> idx((size_t)steps * batch);
+    uint64_t z = seed;
+    for (int32_t &v : idx) {
+        z += 0x9e3779b97f4a7c15ULL;
+        uint64_t q = z;
+        q = (q ^ (q >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        q = (q ^ (q >> 27)) * 0x94d049bb133111ebULL;
+        q ^= q >> 31;
+        v = (int32_t)(q % (uint64_t)n_pairs);
+    }
+    int32_t *d_idx = nullptr;
+    float *d_losses = nullptr;
+    MSBG_HIP(hipMalloc(&d_idx, idx.size() * 4));
+    hipError_t e = hipMalloc(&d_losses, (size_t)steps * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, s);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int st = 0; st < steps && e == hipSuccess; ++st) {
+        loss_grad(n, s, d_x0, d_target, d_idx + (size_t)st * batch, batch, d_losses + st);
+        n->adam_t += 1;
+        const float c1 = 1.0f - std::pow(b1, (float)n->adam_t), c2 = 1.0f - std::pow(b2, (float)n->adam_t);
+        adam_kernel<<<blocks_for(kParams), 256, 0, s>>>(n->d_params, n->d_m, n->d_v, n->d_grad, kParams, lr, b1, b2,
+                                                        eps, c1, c2);
+        prep(n, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host_losses, d_losses, (size_t)steps * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d_idx);
+    hipFree(d_losses);
+    MSBG_HIP(e);
+    return MSBG_OK;
+}
+
+int msbg_poisson_net_input(msbg_poisson *p, const msbg_grid *phi, const float *d_r, float inv_scale,
+                           uint16_t *d_x0) {
+    MSBG_CHECK(p && phi && d_r && d_x0, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(phi->L == 4 && phi->sx == p->dom.sx && phi->sy == p->dom.sy && phi->sz == p->dom.sz, MSBG_EINVAL,
+               "phase field grid must have the layout's dimensions and 16^3 blocks");
+    MSBG_CHECK(p->dom.sx >= 2 && p->dom.sy >= 2 && p->dom.sz >= 2, MSBG_EINVAL, "domain must be >= 2 cells per axis");
+    if (!p->n_active) return MSBG_OK;
+    net_input_kernel<<<blocks_for(p->n_active * 8000), 256, 0, stream()>>>(
+        p->d_blocks, p->dom, (int)p->nx, (int)p->ny, p->d_b2s, p->d_listed, d_r, nullptr, inv_scale, phi->d_slab,
+        phi->d_b2s, d_x0, (int)p->n_active);
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_error_target(msbg_poisson *p, const float *d_x, const float *d_xstar, float inv_scale,
+                              float *d_target) {
+    MSBG_CHECK(p && d_x && d_xstar && d_target, MSBG_EINVAL, "null argument");
+    if (!p->n_active) return MSBG_OK;
+    error_target_kernel<<<blocks_for(p->n_active * 4096), 256, 0, stream()>>>(p->d_blocks, p->dom, d_x, d_xstar,
+                                                                             inv_scale, d_target, (int)p->n_active);
+    MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_learned_correct(msbg_poisson *p, msbg_learned *net, const msbg_grid *phi, float *x, const float *b,
+                                 double *r2_before, double *r2_after, int *accepted) {
+    MSBG_CHECK(p && net && phi && x && b && r2_before && r2_after && accepted, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(phi->L == 4 && phi->sx == p->dom.sx && phi->sy == p->dom.sy && phi->sz == p->dom.sz, MSBG_EINVAL,
+               "phase field grid must have the layout's dimensions and 16^3 blocks");
+    MSBG_CHECK(p->dom.sx >= 2 && p->dom.sy >= 2 && p->dom.sz >= 2, MSBG_EINVAL, "domain must be >= 2 cells per axis");
+    *accepted = 0;
+    if (!p->n_active) return MSBG_OK;
+    if (!p->d_r) MSBG_HIP(hipMalloc(&p->d_r, (size_t)p->n_slots * 4096 * 4));
+    pband::residual(p, stream(), MSBG_STORAGE_F32, x, b, p->d_r);
+    MSBG_HIP(hipGetLastError());
+    double b2 = 0.0;
+    int rc = pband::read_norms(p, r2_before, &b2);
+    if (rc) return rc;
+    return learned::correct(p, net, phi, x, b, *r2_before, r2_after, &b2, accepted);
+}
+
+}  // extern "C"

@@ -18,7 +18,7 @@
 //   x_c' = fmaf(omega / 6, t, x_c)        rounded to the storage type (RNE)
 #include <cmath>
 
-#include "internal.hpp"
+#include "poisson.hpp"
 
 namespace msbg {
 namespace {
@@ -58,22 +58,6 @@ struct Sto<MSBG_STORAGE_BF16> {
 };
 
 int elem_bytes(int storage) { return storage == MSBG_STORAGE_F32 ? 4 : 2; }
-
-// One listed value block: its slot, voxel origin and the slots of its six
-// face neighbours (-x, +x, -y, +y, -z, +z; unused where the face is on the
-// domain boundary).
-struct PBlock {
-    int32_t slot, ox, oy, oz;
-    int32_t nb[6];
-};
-
-struct Dom {
-    int32_t sx, sy, sz;
-};
-
-// A work-group covers 4 z-slices of one block: 256 threads = 16 x 16 columns.
-constexpr int kThreads = 256;
-constexpr int kSlices = 4;
 
 struct Cell {
     i64 c, xl, xr, yd, yu, zb, zf;
@@ -260,29 +244,36 @@ __global__ void convert_kernel(const typename Sto<SS>::T *src, typename Sto<DS>:
 
 using namespace msbg;
 
-struct msbg_poisson {
-    Dom dom{};
-    int64_t n_slots = 0, n_active = 0, n_cells = 0;
-    int n_wg = 0;
-    PBlock *d_blocks = nullptr;
-    double2 *d_partial = nullptr;
-    double *d_norm = nullptr;   // [2]
-    float *d_scale = nullptr;   // [2]
-    double *h_norm = nullptr;   // pinned [2]
-    float *d_r = nullptr;       // FP32 residual, n_slots * 4096
-    void *d_e = nullptr, *d_rt = nullptr;  // inner fields (2 or 4 B elements)
-    int inner_bytes = 0;
-    bool e_zeroed = false;
-    // graph of one check period, keyed by its arguments
-    hipStream_t cap_stream = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int g_mode = -1, g_storage = -1, g_sweeps = -1;
-    float g_omega = 0.0f;
-    const void *g_x = nullptr, *g_b = nullptr;
-};
-
 namespace msbg {
+namespace pband {
+
+void residual(msbg_poisson *p, hipStream_t s, int st, const void *x, const void *b, float *r) {
+    if (st == MSBG_STORAGE_F32)
+        residual_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)x,
+                                                                        (const float *)b, r, p->d_partial);
+    else if (st == MSBG_STORAGE_F16)
+        residual_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(
+            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
+    else
+        residual_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(
+            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
+    reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, p->n_wg, (double)p->n_cells, p->d_norm, p->d_scale);
+}
+
+int read_norms(msbg_poisson *p, double *r2, double *b2) {
+    MSBG_HIP(hipMemcpyAsync(p->h_norm, p->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *r2 = p->h_norm[0];
+    *b2 = p->h_norm[1];
+    return MSBG_OK;
+}
+
+}  // namespace pband
+
 namespace {
+
+using pband::read_norms;
+using pband::residual;
 
 void drop_graph(msbg_poisson *p) {
     if (p->exec) hipGraphExecDestroy(p->exec);
@@ -324,19 +315,6 @@ void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, f
         relax(p, s, st, x, x, b, omega, 0, true);
         relax(p, s, st, x, x, b, omega, 1, true);
     }
-}
-
-void residual(msbg_poisson *p, hipStream_t s, int st, const void *x, const void *b, float *r) {
-    if (st == MSBG_STORAGE_F32)
-        residual_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)x,
-                                                                        (const float *)b, r, p->d_partial);
-    else if (st == MSBG_STORAGE_F16)
-        residual_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(
-            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
-    else
-        residual_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(
-            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
-    reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, p->n_wg, (double)p->n_cells, p->d_norm, p->d_scale);
 }
 
 void ir_scale(msbg_poisson *p, hipStream_t s, int st) {
@@ -397,7 +375,7 @@ void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, con
             int sweeps) {
     if (mode == 0) {
         rb_sweeps(p, s, MSBG_STORAGE_F32, x, b, omega, sweeps);
-        residual(p, s, MSBG_STORAGE_F32, x, b, nullptr);
+        residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
     } else {
         ir_scale(p, s, storage);
         rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps);
@@ -427,14 +405,6 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
         p->g_b = b;
     }
     MSBG_HIP(hipGraphLaunch(p->exec, stream()));
-    return MSBG_OK;
-}
-
-int read_norms(msbg_poisson *p, double *r2, double *b2) {
-    MSBG_HIP(hipMemcpyAsync(p->h_norm, p->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    MSBG_HIP(hipStreamSynchronize(stream()));
-    *r2 = p->h_norm[0];
-    *b2 = p->h_norm[1];
     return MSBG_OK;
 }
 
@@ -485,10 +455,21 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
     }
     p->n_cells = cells;
     p->n_wg = (int)(n_active * kSlices);
+    p->nx = layout->nx;
+    p->ny = layout->ny;
+    p->nz = layout->nz;
+    p->n_blocks = layout->n_blocks;
+    std::vector<uint8_t> listed((size_t)layout->n_blocks, 0);
+    for (int64_t k = 0; k < n_active; ++k) listed[active[k]] = 1;
     hipError_t e = hipSuccess;
     if (n_active) e = hipMalloc(&p->d_blocks, sizeof(PBlock) * n_active);
     if (e == hipSuccess && n_active)
         e = hipMemcpy(p->d_blocks, host.data(), sizeof(PBlock) * n_active, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_b2s, sizeof(int32_t) * layout->n_blocks);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_b2s, layout->b2s_host.data(), sizeof(int32_t) * layout->n_blocks, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_listed, layout->n_blocks);
+    if (e == hipSuccess) e = hipMemcpy(p->d_listed, listed.data(), layout->n_blocks, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&p->d_partial, sizeof(double2) * std::max(1, p->n_wg));
     if (e == hipSuccess) e = hipMalloc(&p->d_norm, 2 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&p->d_scale, 2 * sizeof(float));
@@ -513,6 +494,11 @@ int msbg_poisson_free(msbg_poisson *p) {
     hipFree(p->d_r);
     hipFree(p->d_e);
     hipFree(p->d_rt);
+    hipFree(p->d_b2s);
+    hipFree(p->d_listed);
+    hipFree(p->d_delta);
+    hipFree(p->d_xtry);
+    hipFree(p->d_dots);
     delete p;
     return MSBG_OK;
 }
@@ -597,21 +583,41 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
         st->converged = 1;
         return MSBG_OK;
     }
+    MSBG_CHECK(!prm->learned || prm->phi, MSBG_EINVAL, "the learned correction needs the phase field grid");
+    if (prm->phi)
+        MSBG_CHECK(prm->phi->L == 4 && prm->phi->sx == p->dom.sx && prm->phi->sy == p->dom.sy &&
+                       prm->phi->sz == p->dom.sz,
+                   MSBG_EINVAL, "phase field grid must have the layout's dimensions and 16^3 blocks");
     const int mode = prm->refine;
     if (mode) {
         rc = ensure_inner(p, prm->storage);
         if (rc) return rc;
+    } else if (!p->d_r) {
+        MSBG_HIP(hipMalloc(&p->d_r, (size_t)p->n_slots * 4096 * 4));
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MSBG_HIP(hipEventCreate(&e0));
     MSBG_HIP(hipEventCreate(&e1));
     MSBG_HIP(hipEventRecord(e0, stream()));
-    residual(p, stream(), MSBG_STORAGE_F32, x, b, mode ? p->d_r : nullptr);
+    residual(p, stream(), MSBG_STORAGE_F32, x, b, p->d_r);
     MSBG_HIP(hipGetLastError());
     double r2 = 0.0, b2 = 0.0;
     rc = read_norms(p, &r2, &b2);
     st->checks = 1;
+    bool corrected = true;  // no correction before the first period
     while (rc == MSBG_OK) {
+        if (prm->learned && !corrected) {
+            const double rel = b2 > 0.0 ? std::sqrt(r2 / b2) : std::sqrt(r2);
+            if (rel > prm->tol && std::isfinite(rel)) {
+                int acc = 0;
+                rc = learned::correct(p, prm->learned, prm->phi, x, b, r2, &r2, &b2, &acc);
+                if (rc) break;
+                st->corrections += 1;
+                st->accepted += acc;
+                st->checks += 1;
+            }
+        }
+        corrected = true;
         const double rel = b2 > 0.0 ? std::sqrt(r2 / b2) : std::sqrt(r2);
         st->r2 = r2;
         st->b2 = b2;
@@ -626,6 +632,7 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
         st->sweeps += prm->sweeps_per_check;
         st->checks += 1;
         rc = read_norms(p, &r2, &b2);
+        corrected = false;
     }
     hipError_t e = hipEventRecord(e1, stream());
     if (e == hipSuccess) e = hipEventSynchronize(e1);

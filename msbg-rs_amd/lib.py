@@ -80,6 +80,18 @@ _SIGS = {
     "msbg_poisson_residual": [vp, i32, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     "msbg_poisson_convert": [i32, vp, i32, vp, i64],
     "msbg_poisson_solve": [vp, vp, vp, vp, vp],
+    "msbg_learned_param_count": [C.POINTER(C.c_int)],
+    "msbg_learned_new": [PP],
+    "msbg_learned_free": [vp],
+    "msbg_learned_set_params": [vp, vp, f32],
+    "msbg_learned_get_params": [vp, vp, C.POINTER(C.c_float)],
+    "msbg_learned_forward": [vp, vp, i64, vp],
+    "msbg_learned_loss_grad": [vp, vp, vp, i64, C.POINTER(C.c_double), vp],
+    "msbg_learned_train": [vp, vp, vp, i64, i32, i32, f32, u64, vp],
+    "msbg_poisson_net_input": [vp, vp, vp, f32, vp],
+    "msbg_poisson_error_target": [vp, vp, vp, f32, vp],
+    "msbg_poisson_learned_correct": [vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int)],
 }
 _RESTYPE = {"msbg_last_error": C.c_char_p}
 
@@ -87,13 +99,15 @@ _RESTYPE = {"msbg_last_error": C.c_char_p}
 class PoissonParams(C.Structure):
     """msbg_poisson_params (include/msbg_hip.h)."""
     _fields_ = [("storage", C.c_int), ("refine", C.c_int), ("omega", C.c_float), ("sweeps_per_check", C.c_int),
-                ("max_sweeps", C.c_int), ("reserved", C.c_int), ("tol", C.c_double)]
+                ("max_sweeps", C.c_int), ("reserved", C.c_int), ("tol", C.c_double), ("learned", C.c_void_p),
+                ("phi", C.c_void_p)]
 
 
 class PoissonStats(C.Structure):
     """msbg_poisson_stats (include/msbg_hip.h)."""
-    _fields_ = [("sweeps", C.c_int), ("checks", C.c_int), ("converged", C.c_int), ("reserved", C.c_int),
-                ("rel_residual", C.c_double), ("r2", C.c_double), ("b2", C.c_double), ("ms", C.c_float)]
+    _fields_ = [("sweeps", C.c_int), ("checks", C.c_int), ("converged", C.c_int), ("corrections", C.c_int),
+                ("accepted", C.c_int), ("reserved", C.c_int), ("rel_residual", C.c_double), ("r2", C.c_double),
+                ("b2", C.c_double), ("ms", C.c_float)]
 
 
 class MsbgError(RuntimeError):

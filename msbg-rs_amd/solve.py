@@ -82,14 +82,36 @@ class PoissonBand:
         return r2.value, b2.value
 
     def solve(self, x, b, storage=Storage.F32, refine=False, omega=1.9, sweeps_per_check=10, max_sweeps=10000,
-              tol=1e-6):
+              tol=1e-6, learned=None, phi=None):
         """Red-black SOR solve of A x = b (x, b FP32 fields; x = initial guess,
         updated in place).  refine=True: FP32 iterative refinement with the
-        inner sweeps in `storage`.  Returns the msbg_poisson_stats fields."""
-        prm = _lib.PoissonParams(storage, int(refine), omega, sweeps_per_check, max_sweeps, 0, tol)
+        inner sweeps in `storage`.  learned (a CorrectorNet) with phi (the
+        phase-field DeviceGrid): a correction is tried after every check
+        period.  Returns the msbg_poisson_stats fields."""
+        prm = _lib.PoissonParams(storage, int(refine), omega, sweeps_per_check, max_sweeps, 0, tol,
+                                 learned.h if learned is not None else None, phi.h if phi is not None else None)
         st = _lib.PoissonStats()
         self.lib.call("msbg_poisson_solve", self.h, C.byref(prm), x.ptr, b.ptr, C.byref(st))
         return {k: getattr(st, k) for k, _ in _lib.PoissonStats._fields_ if k != "reserved"}
+
+    def net_input(self, phi, r, inv_scale):
+        """Network input of every listed block (device bf16 bits, n x 20^3 x 2)."""
+        x0 = self.lib.empty((self.n_active, 20, 20, 20, 2), np.uint16)
+        self.lib.call("msbg_poisson_net_input", self.h, phi.h, r.ptr, inv_scale, x0.ptr)
+        return x0
+
+    def error_target(self, x, xstar, inv_scale, out=None):
+        """(x* - x) * inv_scale on every listed block (device float32, n x 4096)."""
+        t = out if out is not None else self.lib.empty((self.n_active, 4096), np.float32)
+        self.lib.call("msbg_poisson_error_target", self.h, x.ptr, xstar.ptr, inv_scale, t.ptr)
+        return t
+
+    def learned_correct(self, net, phi, x, b):
+        """One learned correction of x; returns (r2_before, r2_after, accepted)."""
+        r2b, r2a, acc = C.c_double(), C.c_double(), C.c_int()
+        self.lib.call("msbg_poisson_learned_correct", self.h, net.h, phi.h, x.ptr, b.ptr, C.byref(r2b), C.byref(r2a),
+                      C.byref(acc))
+        return r2b.value, r2a.value, bool(acc.value)
 
     def close(self):
         if self.h:
