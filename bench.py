@@ -55,13 +55,44 @@ def build_scene(pkg, lib):
     return sc
 
 
-def cpu_baseline(sc, dt_mc, dt_bl, reps):
-    """The oracle's C++ restatement of the reference sweeper (thread-local halo
-    tiles, OpenMP over the blocks of a colour), FTZ/DAZ as the reference benches."""
+def fp64_error(orc, g, bids, b, xs):
+    """Final-field error of each solve variant against the FP64 CG solution
+    (the oracle, ||b - A x*|| <= 1e-11 ||b||) over the listed cells."""
+    xd, it = oracle_cg64_solution(orc, g, bids, b)
+    mask = np.zeros((g.n_slots, 4096), bool)
+    mask[g.b2s[bids]] = True
+    ref = xd[mask]
+    out = {"cg64_iterations": it}
+    for name, x in xs.items():
+        d = x.reshape(-1, 4096)[mask].astype(np.float64) - ref
+        out[name] = {"max_rel": float(np.abs(d).max() / np.abs(ref).max()),
+                     "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref))}
+    return out
+
+
+def oracle_cg64_solution(orc, g, bids, b):
+    x = np.zeros(g.n_slots * 4096, np.float64)
+    bb = np.ascontiguousarray(b, np.float64).reshape(-1)
+    it = orc.orc_cg64(g.sx, g.sy, g.sz, x, bb, g.n_slots, g.b2s, np.ascontiguousarray(bids, np.int64), bids.size,
+                      1e-11, 20000)
+    return x.reshape(-1, 4096), it
+
+
+def cpu_baseline(sc, dt_mc, dt_bl, reps, solves):
+    """CPU leg.  Baseline: the oracle's C++ restatement of the reference
+    sweeper (thread-local halo tiles, OpenMP over the blocks of a colour; FTZ /
+    DAZ off, no denormals arise in this field).  Also the FP64 CG oracle
+    solutions the GPU solves are compared with (`solves`: label -> (grid,
+    blocks, rhs slab, {variant: x})."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
 
     orc = oracle_ffi.load()
+    errors = {}
+    for label, (g, bids, b, xs) in solves.items():
+        t = time.time()
+        errors[label] = fp64_error(orc, g, bids, b, xs)
+        log("fp64 oracle %s: %.1fs" % (label, time.time() - t))
     fine, coarse = sc.grids[0], sc.grids[1]
     active = sc.level_blocks(0)
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -78,33 +109,44 @@ def cpu_baseline(sc, dt_mc, dt_bl, reps):
     cells = active.size * 4096 * 2
     return {"value": cells / best, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": "full workload: one MC19 + one BiLap25 8-colour multires sweep over %d level-0 blocks, "
-                      "best of %d after 1 warm-up, FTZ/DAZ off" % (active.size, reps)}
+                      "best of %d after 1 warm-up, FTZ/DAZ off" % (active.size, reps)}, errors
 
 
 def particle_leg(pkg, lib, sc, n, steps, warmup):
     """16M particles uniform in the level-0 band: stable 30-bit Morton sort,
     then quadratic B-spline P2G (sort + voxel ranges + gather) and G2P, each
     timed with HIP events around its call (inputs resident in HBM)."""
-    import ctypes as C
     from msbg_rs_amd import scenes
-    from msbg_rs_amd import transfer as T
 
     t = time.time()
     pos, vel = scenes.band_particles(sc, n)
     log("particles: %d generated in %.1fs" % (n, time.time() - t))
-    dpos, dvel = lib.to_device(pos), lib.to_device(vel)
     cap = 3 * int((sc.levels == 0).sum()) + 4096
-    tg = T.TransferGrid(sc.S, sc.S, sc.S, capacity_blocks=cap, lib=lib)
+    out = transfer_timing(lib, (sc.S,) * 3, pos, vel, 2, cap, steps, warmup)
+    out["note"] = ("quadratic B-spline P2G (includes its own Morton sort, active-block build and voxel ranges) "
+                   "+ G2P, cell-centred 16^3-block grid over the 512^3 domain")
+    return out
+
+
+def transfer_timing(lib, dims, pos, vel, order, cap, steps, warmup):
+    """Morton sort, B-spline P2G and G2P of resident particles, each timed with
+    a HIP event pair around its call."""
+    import ctypes as C
+    from msbg_rs_amd import transfer as T
+
+    n = pos.shape[0]
+    dpos, dvel = lib.to_device(pos), lib.to_device(vel)
+    tg = T.TransferGrid(*dims, capacity_blocks=cap, lib=lib)
     keys, perm = lib.empty(n, np.uint32), lib.empty(n, np.uint32)
     out = lib.empty((n, 3), np.float32)
     handles = (C.c_void_p * 3)(*[g.h for g in tg.velocity])
     k = C.c_int64()
 
     def p2g():
-        lib.call("msbg_p2g_bspline", 2, tg.mass.h, handles, dpos.ptr, dvel.ptr, None, n, C.byref(k))
+        lib.call("msbg_p2g_bspline", order, tg.mass.h, handles, dpos.ptr, dvel.ptr, None, n, C.byref(k))
 
     def g2p():
-        lib.call("msbg_g2p_bspline", 2, handles, dpos.ptr, n, out.ptr)
+        lib.call("msbg_g2p_bspline", order, handles, dpos.ptr, n, out.ptr)
 
     def sort():
         lib.call("msbg_morton_sort", dpos.ptr, n, keys.ptr, perm.ptr)
@@ -126,13 +168,72 @@ def particle_leg(pkg, lib, sc, n, steps, warmup):
     step_ms = ms["p2g"] + ms["g2p"]
     return {
         "particles": n,
+        "order": order,
         "active_blocks": k.value,
         "ms": ms,
         "morton_sort_particles_per_s": n / (ms["sort"] * 1e-3),
         "p2g_g2p_particles_per_s": n / (step_ms * 1e-3),
-        "note": "quadratic B-spline P2G (includes its own Morton sort, active-block build and voxel ranges) "
-                "+ G2P, cell-centred 16^3-block grid over the 512^3 domain",
     }
+
+
+SOLVE_VARIANTS = (("f32_plain", 0, False, False), ("f32_learned", 0, False, True), ("f16_refine", 1, True, False),
+                  ("bf16_refine", 2, True, False), ("f16_refine_learned", 1, True, True),
+                  ("bf16_refine_learned", 2, True, True))
+
+
+def solve_leg(pkg, lib, band, phi, b, reps, keep_x=False):
+    """Time to ||b - A x|| / ||b|| <= 1e-6 from x = 0 (red-black SOR, omega 1.9,
+    FP64 residual check every 4 sweeps) for each variant; best of `reps`
+    (msbg_poisson_stats.ms: HIP events around the whole solve)."""
+    from msbg_rs_amd.learned import WEIGHTS
+    net = pkg.CorrectorNet.load(WEIGHTS, lib=lib)
+    out, xs = {}, {}
+    for name, storage, refine, learned in SOLVE_VARIANTS:
+        best = None
+        for _ in range(reps):
+            x = band.field(0)
+            st = band.solve(x, b, storage=storage, refine=refine, omega=1.9, sweeps_per_check=4, max_sweeps=20000,
+                            tol=1e-6, learned=net if learned else None, phi=phi if learned else None)
+            if best is None or st["ms"] < best["ms"]:
+                best, xbest = st, x
+        out[name] = {k: best[k] for k in ("ms", "sweeps", "checks", "corrections", "accepted", "rel_residual",
+                                          "converged")}
+        if keep_x:
+            xs[name] = xbest.get()
+    base = out["f32_plain"]["ms"]
+    for v in out.values():
+        v["time_vs_f32_plain"] = v["ms"] / base
+    return out, xs
+
+
+def shell_legs(pkg, lib, n_particles, steps, warmup, reps):
+    """The 128^3 shell (280 blocks): 20 weighted-Jacobi sweeps (omega 2/3) of
+    the phase field, trilinear P2G / G2P of 1M particles in the shell, and the
+    Poisson solve variants to 1e-6."""
+    from msbg_rs_amd import scenes
+    g, bids, centre = scenes.shell_scene()
+    phi = pkg.DeviceGrid.from_host(g, lib=lib)
+    band = pkg.PoissonBand(phi, bids, lib=lib)
+    slab = np.ascontiguousarray(g.slab, np.float32).ravel()
+    x, tmp, zero = band.field(0, slab), band.field(0, slab), band.field(0)
+    for _ in range(warmup):
+        band.jacobi_sweeps(0, x, tmp, zero, 2.0 / 3.0, 20)
+    e0, e1 = lib.event(), lib.event()
+    lib.call("msbg_event_record", e0)
+    for _ in range(steps):
+        band.jacobi_sweeps(0, x, tmp, zero, 2.0 / 3.0, 20)
+    lib.call("msbg_event_record", e1)
+    lib.synchronize()
+    ms = lib.elapsed_ms(e0, e1) / steps
+    cells = band.n_cells
+    jac = {"cells": cells, "ms_per_20_sweeps": ms, "cell_updates_per_s": 20 * cells / (ms * 1e-3),
+           "algorithmic_GBs": 12.0 * 20 * cells / (ms * 1e-3) / 1e9, "bytes_per_cell_update": 12,
+           "note": "x -> tmp -> x ping-pong, b = 0 read, one HIP graph of 20 launches"}
+    pos, vel = scenes.shell_particles(n_particles, centre)
+    tri = transfer_timing(lib, (128, 128, 128), pos, vel, 1, 4 * bids.size + 512, steps, warmup)
+    b = lib.to_device(scenes.poisson_rhs(g, bids))
+    solve, xs = solve_leg(pkg, lib, band, phi, b, reps, keep_x=True)
+    return {"blocks": int(bids.size), "jacobi20": jac, "trilinear_p2g_g2p": tri, "solve": solve}, (g, bids, xs)
 
 
 def main():
@@ -144,6 +245,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--particles", type=int, default=16 * 1024 * 1024)
     ap.add_argument("--particle-steps", type=int, default=5)
+    ap.add_argument("--shell-particles", type=int, default=1 << 20)
+    ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--fp64-band", action="store_true",
+                    help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the sparse-grid path is not sharded across GPUs "
@@ -240,9 +345,23 @@ def main():
     if args.particles > 0:
         log("particle leg ...")
         out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)
+    from msbg_rs_amd import scenes
+    log("128^3 shell legs ...")
+    out["shell_128"], (g1, bids1, xs1) = shell_legs(pkg, lib, args.shell_particles, 20, 3, args.solve_reps)
+    log("512^3 band solve ...")
+    rhs512 = scenes.poisson_rhs(sc.grids[0], active)
+    band512 = pkg.PoissonBand(fine, active, lib=lib)
+    solve512, xs512 = solve_leg(pkg, lib, band512, fine, lib.to_device(rhs512), args.solve_reps,
+                                keep_x=args.fp64_band and not args.no_cpu)
+    out["solve_512"] = {"blocks": int(active.size), "cells": band512.n_cells, "variants": solve512,
+                        "note": "7-point Poisson on the level-0 band, zero Dirichlet outside it, LCG right-hand "
+                                "side; red-black SOR omega 1.9, FP64 residual check every 4 sweeps"}
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(sc, dt_mc, dt_bl, args.cpu_reps)
+        solves = {"shell_128": (g1, bids1, scenes.poisson_rhs(g1, bids1), xs1)}
+        if args.fp64_band:
+            solves["band_512"] = (sc.grids[0], active, rhs512, xs512)
+        out["cpu_baseline"], out["fp64_error"] = cpu_baseline(sc, dt_mc, dt_bl, args.cpu_reps, solves)
     print(json.dumps(out), flush=True)
 
 

@@ -273,6 +273,11 @@ int msbg_poisson_info(const msbg_poisson *p, int64_t *n_active, int64_t *n_cells
 /* One weighted Jacobi sweep x_out = x_in + omega/6 (b - A x_in) over the listed
  * cells (x_in != x_out; other cells of x_out are not written). */
 int msbg_poisson_jacobi(msbg_poisson *p, int storage, const void *x_in, void *x_out, const void *b, float omega);
+/* `sweeps` (even) weighted Jacobi sweeps x -> tmp -> x ..., replayed as one
+ * HIP graph; tmp must hold x's values outside the listed cells (e.g. a copy
+ * of x), the result is in x. */
+int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp, const void *b, float omega,
+                               int sweeps);
 /* One red-black SOR half sweep in place over the listed cells with
  * (x + y + z) & 1 == color. */
 int msbg_poisson_rbgs(msbg_poisson *p, int storage, void *x, const void *b, int color, float omega);

@@ -49,6 +49,11 @@ struct msbg_poisson {
     int g_mode = -1, g_storage = -1, g_sweeps = -1;
     float g_omega = 0.0f;
     const void *g_x = nullptr, *g_b = nullptr;
+    // graph of a run of Jacobi sweeps, keyed by its arguments
+    hipGraphExec_t jexec = nullptr;
+    int j_storage = -1, j_sweeps = -1;
+    float j_omega = 0.0f;
+    const void *j_x = nullptr, *j_t = nullptr, *j_b = nullptr;
 };
 
 namespace msbg {

@@ -485,6 +485,7 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
 int msbg_poisson_free(msbg_poisson *p) {
     if (!p) return MSBG_OK;
     drop_graph(p);
+    if (p->jexec) hipGraphExecDestroy(p->jexec);
     if (p->cap_stream) hipStreamDestroy(p->cap_stream);
     hipFree(p->d_blocks);
     hipFree(p->d_partial);
@@ -519,6 +520,40 @@ int msbg_poisson_jacobi(msbg_poisson *p, int storage, const void *x_in, void *x_
     if (!p->n_active) return MSBG_OK;
     relax(p, stream(), storage, x_in, x_out, b, omega, 0, false);
     MSBG_HIP(hipGetLastError());
+    return MSBG_OK;
+}
+
+int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp, const void *b, float omega,
+                               int sweeps) {
+    MSBG_CHECK(p && x && tmp && b, MSBG_EINVAL, "null argument");
+    MSBG_CHECK(x != tmp, MSBG_EINVAL, "Jacobi needs distinct fields");
+    MSBG_CHECK(sweeps >= 0 && sweeps % 2 == 0, MSBG_EINVAL, "sweeps must be even (the result returns to x)");
+    int rc = check_storage(storage);
+    if (rc) return rc;
+    if (!p->n_active || !sweeps) return MSBG_OK;
+    if (!(p->jexec && p->j_storage == storage && p->j_sweeps == sweeps && p->j_omega == omega && p->j_x == x &&
+          p->j_t == tmp && p->j_b == b)) {
+        if (p->jexec) hipGraphExecDestroy(p->jexec);
+        p->jexec = nullptr;
+        if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < sweeps; i += 2) {
+            relax(p, p->cap_stream, storage, x, tmp, b, omega, 0, false);
+            relax(p, p->cap_stream, storage, tmp, x, b, omega, 0, false);
+        }
+        hipGraph_t graph = nullptr;
+        MSBG_HIP(hipStreamEndCapture(p->cap_stream, &graph));
+        hipError_t e = hipGraphInstantiate(&p->jexec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        MSBG_HIP(e);
+        p->j_storage = storage;
+        p->j_sweeps = sweeps;
+        p->j_omega = omega;
+        p->j_x = x;
+        p->j_t = tmp;
+        p->j_b = b;
+    }
+    MSBG_HIP(hipGraphLaunch(p->jexec, stream()));
     return MSBG_OK;
 }
 

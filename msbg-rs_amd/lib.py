@@ -77,6 +77,7 @@ _SIGS = {
     "msbg_poisson_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "msbg_poisson_jacobi": [vp, i32, vp, vp, vp, f32],
     "msbg_poisson_rbgs": [vp, i32, vp, vp, i32, f32],
+    "msbg_poisson_jacobi_sweeps": [vp, i32, vp, vp, vp, f32, i32],
     "msbg_poisson_residual": [vp, i32, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     "msbg_poisson_convert": [i32, vp, i32, vp, i64],
     "msbg_poisson_solve": [vp, vp, vp, vp, vp],

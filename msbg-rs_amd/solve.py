@@ -71,6 +71,11 @@ class PoissonBand:
     def jacobi(self, storage, x_in, x_out, b, omega):
         self.lib.call("msbg_poisson_jacobi", self.h, storage, x_in.ptr, x_out.ptr, b.ptr, omega)
 
+    def jacobi_sweeps(self, storage, x, tmp, b, omega, sweeps):
+        """`sweeps` (even) Jacobi sweeps x -> tmp -> x as one HIP graph; tmp
+        must hold x's values outside the listed cells."""
+        self.lib.call("msbg_poisson_jacobi_sweeps", self.h, storage, x.ptr, tmp.ptr, b.ptr, omega, sweeps)
+
     def rbgs(self, storage, x, b, color, omega):
         self.lib.call("msbg_poisson_rbgs", self.h, storage, x.ptr, b.ptr, color, omega)
 
