@@ -247,6 +247,10 @@ def main():
     ap.add_argument("--particle-steps", type=int, default=5)
     ap.add_argument("--shell-particles", type=int, default=1 << 20)
     ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--only-smoother", action="store_true",
+                    help="time the smoother step only (for counter runs): no particle, shell, solve or CPU legs")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="first run a plain FP32 copy kernel over MB MiB three times (counter calibration)")
     ap.add_argument("--fp64-band", action="store_true",
                     help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
     args = ap.parse_args()
@@ -259,6 +263,16 @@ def main():
     lib = pkg.lib.load()
     if lib.device_count() < 1:
         raise RuntimeError("bench.py needs a HIP device")
+    if args.calibrate_copy > 0:
+        # convert_kernel<f32, f32>: reads and writes exactly 4 B per element
+        n = args.calibrate_copy * (1 << 20) // 4
+        src, dst = lib.empty(n, np.float32), lib.empty(n, np.float32)
+        for _ in range(3):
+            lib.call("msbg_poisson_convert", 0, src.ptr, 0, dst.ptr, n)
+        lib.synchronize()
+        log("copy calibration: 3 x %d bytes read + written" % (4 * n))
+        src.free()
+        dst.free()
     sc = build_scene(pkg, lib)
     fine = pkg.DeviceGrid.from_host(sc.grids[0], lib=lib)
     coarse = pkg.DeviceGrid.from_host(sc.grids[1], lib=lib)
@@ -342,6 +356,9 @@ def main():
         },
         "kernel_ms": {"mean_curvature": k_ms[1], "bilaplacian": k_ms[2]},
     }
+    if args.only_smoother:
+        print(json.dumps(out), flush=True)
+        return
     if args.particles > 0:
         log("particle leg ...")
         out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)
