@@ -176,6 +176,43 @@ def transfer_timing(lib, dims, pos, vel, order, cap, steps, warmup):
     }
 
 
+def scale_up_leg(pkg, lib, S, steps):
+    """The reference's shell generator (benches/allocator_benches.rs:233-256:
+    blocks whose centre lies in [0.9, 1.0] * min(n) / 2) at S^3, sized past the
+    256 MB Infinity Cache so HBM is the bound: 8-colour in-place sweeps of each
+    stencil, HIP events around each replayed sweep graph."""
+    from msbg_rs_amd import scenes
+    nb = S // 16
+    bids = scenes.reference_shell_blocks(nb, nb, nb)
+    g = pkg.DeviceGrid(S, S, S, 4, 0.0, 1.0, capacity_blocks=bids.size + 8, lib=lib)
+    g.ensure_blocks_parallel(bids)
+    chunk = 4096
+    for k in range(0, bids.size, chunk):
+        part = bids[k:k + chunk]
+        vals = scenes.lcg_uniform(7 + k, part.size * 4096).reshape(part.size, 4096)
+        g.write_blocks(part, vals)
+    plan = pkg.SweepPlan(g, bids)
+    cells = bids.size * 4096
+    out = {"domain": S, "blocks": int(bids.size), "field_bytes": cells * 4}
+    e0, e1 = lib.event(), lib.event()
+    for name, st, dt in (("laplacian7", pkg.Stencil.Laplacian, 0.1), ("meancurv19", pkg.Stencil.MeanCurvature, 0.02),
+                         ("bilaplacian25", pkg.Stencil.BiLaplacian, 0.005)):
+        plan.run_graph_async(st, dt, 1)
+        lib.synchronize()
+        lib.call("msbg_event_record", e0)
+        for _ in range(steps):
+            plan.run_graph_async(st, dt, 1)
+        lib.call("msbg_event_record", e1)
+        lib.synchronize()
+        ms = lib.elapsed_ms(e0, e1) / steps
+        gbs = 8.0 * cells / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_sweep": ms, "cell_updates_per_s": cells / (ms * 1e-3), "achieved_GBs": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    plan.free()
+    g.free()
+    return out
+
+
 SOLVE_VARIANTS = (("f32_plain", 0, False, False), ("f32_learned", 0, False, True), ("f16_refine", 1, True, False),
                   ("bf16_refine", 2, True, False), ("f16_refine_learned", 1, True, True),
                   ("bf16_refine_learned", 2, True, True))
@@ -251,6 +288,8 @@ def main():
                     help="time the smoother step only (for counter runs): no particle, shell, solve or CPU legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="first run a plain FP32 copy kernel over MB MiB three times (counter calibration)")
+    ap.add_argument("--scale-up", type=int, default=1536, metavar="S",
+                    help="domain edge of the HBM-bound scale-up sweep leg (0 = skip)")
     ap.add_argument("--fp64-band", action="store_true",
                     help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
     args = ap.parse_args()
@@ -359,6 +398,9 @@ def main():
     if args.only_smoother:
         print(json.dumps(out), flush=True)
         return
+    if args.scale_up > 0:
+        log("scale-up leg ...")
+        out["scale_up"] = scale_up_leg(pkg, lib, args.scale_up, 3)
     if args.particles > 0:
         log("particle leg ...")
         out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)
