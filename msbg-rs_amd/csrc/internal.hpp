@@ -69,6 +69,17 @@ struct msbg_grid {
 // ---------------------------------------------------------------------------
 namespace msbg {
 
+// Work-group index -> work item, XCD-aware: the dispatcher hands work-group i
+// to XCD i % 8, so give each XCD a contiguous run of the work list (blocks in
+// Morton order), letting neighbouring blocks share that XCD's L2.  A bijection
+// on [0, n) whatever the dispatch order is.
+__device__ __forceinline__ int xcd_remap(int i, int n) {
+    constexpr int kXcds = 8;
+    const int q = n / kXcds, r = n % kXcds;
+    const int xcd = i % kXcds, k = i / kXcds;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
 __device__ __forceinline__ int reflect_axis(int g, int max) {
     int period = 2 * (max + 1);
     int r = g % period;

@@ -261,10 +261,11 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridVi
     __shared__ __attribute__((aligned(16))) float tile[HSX * HSX * kPitch];
     __shared__ int s_nb[NB];
     __shared__ int s_nbc[MULTI ? NB : 1];
-    const int32_t *row = topo + (int64_t)blockIdx.x * NB;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int32_t *row = topo + (int64_t)wg * NB;
     const int slot = row[CENTER];
     if (slot < 2) return;  // dummy or absent: solver.rs:201-204 (uniform per workgroup)
-    const int64_t bid = bids[blockIdx.x];
+    const int64_t bid = bids[wg];
     if (threadIdx.x < NB) {
         const int code = row[threadIdx.x];
         s_nb[threadIdx.x] = code >= 0 ? code : 0;
