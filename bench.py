@@ -209,6 +209,28 @@ def scale_up_leg(pkg, lib, S, steps):
         out[name] = {"ms_per_sweep": ms, "cell_updates_per_s": cells / (ms * 1e-3), "achieved_GBs": gbs,
                      "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
     plan.free()
+    # weighted Jacobi (omega 2/3, 7-point, b = 0 read): one launch per sweep over all blocks
+    band = pkg.PoissonBand(g, bids, lib=lib)
+    x = lib.empty(int(g.dims()[7]) * 4096, np.float32)
+    lib.call("msbg_memcpy_dtod", x.ptr, g.views()[0], x.nbytes)
+    tmp = lib.empty(x.shape, np.float32)
+    lib.call("msbg_memcpy_dtod", tmp.ptr, x.ptr, x.nbytes)
+    zero = lib.empty(x.shape, np.float32)
+    lib.call("msbg_memcpy_htod", zero.ptr, np.zeros(x.shape, np.float32).ctypes.data, x.nbytes)
+    band.jacobi_sweeps(0, x, tmp, zero, 2.0 / 3.0, 2)
+    lib.synchronize()
+    lib.call("msbg_event_record", e0)
+    for _ in range(steps):
+        band.jacobi_sweeps(0, x, tmp, zero, 2.0 / 3.0, 2)
+    lib.call("msbg_event_record", e1)
+    lib.synchronize()
+    ms = lib.elapsed_ms(e0, e1) / (2 * steps)
+    gbs = 12.0 * cells / (ms * 1e-3) / 1e9
+    out["jacobi7"] = {"ms_per_sweep": ms, "cell_updates_per_s": cells / (ms * 1e-3), "achieved_GBs": gbs,
+                      "frac_of_hbm_peak": gbs / HBM_PEAK_GBS, "bytes_per_cell_update": 12}
+    band.close()
+    for a in (x, tmp, zero):
+        a.free()
     g.free()
     return out
 

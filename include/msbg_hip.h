@@ -225,6 +225,23 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
 /* The matching G2P: d_out (n x 3) = sum over the support cells of w * vel. */
 int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64_t n, float *d_out);
 
+/* particles::splat -- particles/splat.rs:92-319 with D = Density (u16,
+ * channel.rs:303-375): the active blocks of the particles (footprint r_scan,
+ * active.rs:15-47) are ensured in the layout grid g and filled with 65535 in
+ * the u16 slab d_slab16 (slab_slots * 4096 elements over g's bid -> slot
+ * table; slot 0 / 1 are set to 0 / 65535, the grid's Density(0) /
+ * Density(u16::MAX) dummies); then every voxel within r_scan of a particle,
+ * in the 16 + 2 ceil(r_scan) window of the particle's block, takes
+ * min(voxel, round_half_away(d^2 / r_scan^2 * 65535)), d measured to the
+ * voxel centre.  A window region (interior / margin slab, splat.rs:263-313)
+ * is committed only when it lies inside the domain and targets a value block;
+ * margin regions are committed over their own width (the reference's 4-lane
+ * margin rows coincide with it for 3 < r_scan <= 4 and spill past it below).
+ * Particles lie in [0, s) per axis; 0 < r_scan <= 6.  *n_active = the number
+ * of active blocks. */
+int msbg_density_splat(msbg_grid *g, uint16_t *d_slab16, int64_t slab_slots, const float *d_pos, int64_t n,
+                       float r_scan, int64_t *n_active);
+
 /* ---- Poisson relaxation on a band of value blocks ------------------------
  * No reference counterpart: the reference's solver is planned, not built
  * (docs/roadmap.md:610-702; the weighted Jacobi formula y = omega (b + sum nb)

@@ -283,12 +283,12 @@ __global__ __launch_bounds__(256) void delta_kernel(const PBlock *__restrict__ b
 // q = A delta (delta is zero outside the listed cells); partials of r . q and q . q.
 __global__ __launch_bounds__(kThreads) void dots_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                         const float *delta, const float *r, double2 *partial) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     double rq = 0.0, qq = 0.0;
     if (gx < d.sx && gy < d.sy) {
-        const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+        const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
         for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
             const int gz = B.oz + tz;
             if (gz >= d.sz) break;
@@ -360,12 +360,12 @@ __global__ __launch_bounds__(256) void reduce_dots_kernel(const double2 *__restr
 // (0 when q = 0); commit = 1 copies x_try back into x instead.
 __global__ __launch_bounds__(kThreads) void try_kernel(const PBlock *__restrict__ blocks, Dom d, const double *dots,
                                                        const float *delta, float *x, float *xt, int commit) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     if (gx >= d.sx || gy >= d.sy) return;
     const float alpha = dots[1] > 0.0 ? (float)(dots[0] / dots[1]) : 0.0f;
-    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
     for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
         if (B.oz + tz >= d.sz) break;
         const int64_t c = ((int64_t)B.slot << 12) + (tx | ty << 4 | tz << 8);

@@ -93,11 +93,11 @@ template <int ST, bool RB>
 __global__ __launch_bounds__(kThreads) void relax_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                          const typename Sto<ST>::T *xin, typename Sto<ST>::T *xout,
                                                          const typename Sto<ST>::T *b, float w6, int color) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     if (gx >= d.sx || gy >= d.sy) return;
-    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
 #pragma unroll
     for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
         const int gz = B.oz + tz;
@@ -117,12 +117,12 @@ __global__ __launch_bounds__(kThreads) void residual_kernel(const PBlock *__rest
                                                             const typename Sto<ST>::T *x,
                                                             const typename Sto<ST>::T *b, float *r,
                                                             double2 *partial) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     double rr = 0.0, bb = 0.0;
     if (gx < d.sx && gy < d.sy) {
-        const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+        const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
         for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
             const int gz = B.oz + tz;
             if (gz >= d.sz) break;
@@ -196,12 +196,12 @@ template <int ST>
 __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__restrict__ blocks, Dom d, const float *r,
                                                             const float *scale, typename Sto<ST>::T *rt,
                                                             typename Sto<ST>::T *e) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     if (gx >= d.sx || gy >= d.sy) return;
     const float inv = scale[1];
-    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
     for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
         if (B.oz + tz >= d.sz) break;
         const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);
@@ -215,12 +215,12 @@ template <int ST>
 __global__ __launch_bounds__(kThreads) void ir_update_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                              const typename Sto<ST>::T *e, const float *scale,
                                                              float *x) {
-    const PBlock B = blocks[blockIdx.x / kSlices];
+    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = B.ox + tx, gy = B.oy + ty;
     if (gx >= d.sx || gy >= d.sy) return;
     const float s = scale[0];
-    const int z0 = (blockIdx.x % kSlices) * (16 / kSlices);
+    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
     for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
         if (B.oz + tz >= d.sz) break;
         const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);

@@ -10,6 +10,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import lib as _lib
 from .lib import DeviceArray, load
 
 
@@ -102,3 +103,31 @@ def gather_rows(src: DeviceArray, perm: DeviceArray, lib=None) -> DeviceArray:
     dst = lib.empty((n,) + src.shape[1:], src.dtype)
     lib.call("msbg_gather_rows", src.ptr, perm.ptr, n, row_bytes, dst.ptr)
     return dst
+
+
+class DensityGrid:
+    """A Density (u16) channel over a 16^3-block layout grid: the layout's
+    bid -> slot table and a device u16 slab with a slot for every block the
+    layout can hold (SparseGrid<Density, 16, 4096>, empty Density(0), full
+    Density(u16::MAX))."""
+
+    def __init__(self, sx, sy, sz, capacity_blocks, lib=None):
+        from .device import DeviceGrid
+        self.lib = lib or _lib.load()
+        self.layout = DeviceGrid(sx, sy, sz, 4, 0.0, 1.0, capacity_blocks=capacity_blocks, lib=self.lib)
+        self.slots = capacity_blocks + 2
+        self.slab = self.lib.empty((self.slots, 4096), np.uint16)
+
+    def splat(self, pos, r_scan):
+        """particles::splat (particles/splat.rs:92-319): returns the number of
+        active blocks."""
+        d = pos if isinstance(pos, DeviceArray) else self.lib.to_device(np.ascontiguousarray(pos, np.float32))
+        k = C.c_int64()
+        self.lib.call("msbg_density_splat", self.layout.h, self.slab.ptr, self.slots, d.ptr, d.shape[0], r_scan,
+                      C.byref(k))
+        return k.value
+
+    def to_host(self):
+        """(b2s, u16 slab[n_slots])"""
+        g = self.layout.to_host()
+        return g.b2s, self.slab.get()[: g.n_slots]

@@ -441,3 +441,109 @@ void orc_p2g_mac_naive(i64 sx, i64 sy, i64 sz, const float *pos, const float *ve
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Density min-splat -- particles/splat.rs:92-319 with D = Density (u16,
+// channel.rs:303-375).  Particles are grouped by the block of their truncated
+// position; each block's group is staged into a (16 + 2m)^3 window (m =
+// ceil(r_scan)) filled with 65535: every voxel ix in [ceil(p - r - 0.5),
+// floor(p + r - 0.5)] per axis (clipped to the window) with
+//   dx = ix - (p.x - 0.5), dz2 = dz*dz, dyz = dz2 + dy*dy, d2 = dx*dx + dyz,
+// d2 <= r^2 takes min(staging, (u16)round_half_away(d2 * (1/r^2) * 65535))
+// (splat.rs:179-225, stage_chunk channel.rs:353-374).  The commit splits the
+// window into 3x3x3 regions (interior 16, margins m); a region is skipped
+// when any of its voxels lies outside the domain or its target block is not a
+// value block (splat.rs:240-292); otherwise each region row commits
+// dst = min(dst, staging) (commit_chunk, channel.rs:337-351).
+// ref_commit_width = 1 reproduces the reference's row width for margin rows
+// (4 lanes, splat.rs:306-309), which spills past the region when m < 4 (into
+// the next row of the -x neighbour, as MAC P2G's margin commit); 0 commits
+// each region over its own width.  For 3 < r_scan <= 4 (the tests' and the
+// demo's r_scan = 4) the two agree.
+// ===========================================================================
+namespace {
+
+uint16_t quant_dsq(float d2, float inv) {
+    const float v = roundf((d2 * inv) * 65535.0f);  // f32::round: half away from zero
+    return (uint16_t)(int32_t)v;
+}
+
+}  // namespace
+
+extern "C" void orc_density_splat(i64 sx, i64 sy, i64 sz, const i32 *b2s, uint16_t *slab, const float *pos, i64 n,
+                                  float r_scan, int ref_commit_width) {
+    const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nz = (sz + 15) >> 4, nxy = nx * ny;
+    const int m = (int)ceilf(r_scan);
+    const int W = 16 + 2 * m;
+    const int MSX = std::max(24, (W + 7) & ~7);  // staging row pitch (the reference's MSX, a multiple of 8)
+    const float dsq_max = r_scan * r_scan, inv = 1.0f / dsq_max;
+    // group particles by block, in bid order (stable)
+    std::vector<i64> bid((size_t)n);
+    for (i64 i = 0; i < n; ++i) {
+        auto tr = [](float v) { return (i64)(v > 0.0f ? (uint64_t)v : 0); };  // `as usize` (saturating)
+        const i64 x = tr(pos[3 * i]), y = tr(pos[3 * i + 1]), z = tr(pos[3 * i + 2]);
+        bid[i] = (x >> 4) + (y >> 4) * nx + (z >> 4) * nxy;
+    }
+    std::vector<i64> order((size_t)n);
+    for (i64 i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](i64 a, i64 b) { return bid[a] < bid[b]; });
+    std::vector<uint16_t> st((size_t)MSX * MSX * MSX + 8);
+    for (i64 k0 = 0; k0 < n;) {
+        i64 k1 = k0;
+        while (k1 < n && bid[order[k1]] == bid[order[k0]]) ++k1;
+        const i64 b = bid[order[k0]];
+        const int bx = (int)(b % nx), by = (int)((b / nx) % ny), bz = (int)(b / nxy);
+        const int b0x = bx * 16, b0y = by * 16, b0z = bz * 16, hi = 16 + m - 1;
+        std::fill(st.begin(), st.end(), (uint16_t)65535);
+        for (i64 k = k0; k < k1; ++k) {
+            const float *p = pos + 3 * order[k];
+            const float px0 = p[0] - 0.5f, py0 = p[1] - 0.5f, pz0 = p[2] - 0.5f;
+            auto lo_of = [&](float c, int b0) { return std::min(std::max((int)ceilf((c - r_scan) - 0.5f), b0 - m), b0 + hi); };
+            auto hi_of = [&](float c, int b0) { return std::min(std::max((int)floorf((c + r_scan) - 0.5f), b0 - m), b0 + hi); };
+            const int ix1 = lo_of(p[0], b0x), ix2 = hi_of(p[0], b0x), iy1 = lo_of(p[1], b0y), iy2 = hi_of(p[1], b0y),
+                      iz1 = lo_of(p[2], b0z), iz2 = hi_of(p[2], b0z);
+            for (int iz = iz1; iz <= iz2; ++iz) {
+                const float dz = (float)iz - pz0, dz2 = dz * dz;
+                for (int iy = iy1; iy <= iy2; ++iy) {
+                    const float dy = (float)iy - py0, dyz = dz2 + dy * dy;
+                    uint16_t *row = &st[((size_t)(iz - b0z + m) * MSX + (iy - b0y + m)) * MSX];
+                    for (int ix = ix1; ix <= ix2; ++ix) {
+                        const float dx = (float)ix - px0, d2 = dx * dx + dyz;
+                        if (d2 <= dsq_max) {
+                            uint16_t &s = row[ix - b0x + m];
+                            s = std::min(s, quant_dsq(d2, inv));
+                        }
+                    }
+                }
+            }
+        }
+        // commit
+        for (int rz = -1; rz <= 1; ++rz)
+            for (int ry = -1; ry <= 1; ++ry)
+                for (int rx = -1; rx <= 1; ++rx) {
+                    auto range = [&](int d, int *lo, int *hi2) {
+                        *lo = d < 0 ? 0 : (d == 0 ? m : m + 16);
+                        *hi2 = d < 0 ? m : (d == 0 ? m + 16 : 2 * m + 16);
+                    };
+                    int xlo, xhi, ylo, yhi, zlo, zhi;
+                    range(rx, &xlo, &xhi);
+                    range(ry, &ylo, &yhi);
+                    range(rz, &zlo, &zhi);
+                    const int gx0 = b0x + xlo - m, gy0 = b0y + ylo - m, gz0 = b0z + zlo - m;
+                    if (gx0 < 0 || gy0 < 0 || gz0 < 0) continue;
+                    if (gx0 + (xhi - xlo) > sx || gy0 + (yhi - ylo) > sy || gz0 + (zhi - zlo) > sz) continue;
+                    const i64 nb = (gx0 >> 4) + (i64)(gy0 >> 4) * nx + (i64)(gz0 >> 4) * nxy;
+                    const i32 slot = b2s[nb];
+                    if (slot < 2) continue;
+                    const int w = (ref_commit_width && rx != 0) ? 4 : (xhi - xlo);
+                    for (int sz_ = zlo; sz_ < zhi; ++sz_)
+                        for (int sy_ = ylo; sy_ < yhi; ++sy_) {
+                            const uint16_t *src = &st[((size_t)sz_ * MSX + sy_) * MSX + xlo];
+                            const i64 d = (i64)slot * 4096 + ((gx0 & 15) | (((gy0 + sy_ - ylo) & 15) << 4) |
+                                                              (((gz0 + sz_ - zlo) & 15) << 8));
+                            for (int k = 0; k < w; ++k) slab[d + k] = std::min(slab[d + k], src[k]);
+                        }
+                }
+        k0 = k1;
+    }
+}

@@ -127,3 +127,39 @@ def test_bspline_p2g_g2p_bitexact(pkg, orc, hip, order, n, s):
     out = tg.g2p(order, pos).get()
     want = oracle_g2p_bspline(orc, order, s, b2s, slabs[1:], pos)
     assert_bitexact(out, want, "g2p")
+
+
+@pytest.mark.parametrize("r_scan,s,n", [(4.0, (32, 32, 32), 1), (4.0, (48, 40, 33), 3000), (2.5, (40, 33, 48), 5000),
+                                        (6.0, (64, 64, 48), 2000), (4.0, (128, 128, 128), 200000)])
+def test_density_splat_bitexact(pkg, orc, hip, r_scan, s, n):
+    """particles::splat for Density against the oracle, bit for bit (min is
+    order-independent): known answer splat_01 for one particle, random clouds
+    with particles on block faces, partial domains, margins 3..6."""
+    from msbg_rs_amd.particles import DensityGrid
+    from test_oracle_splat import density_grid, run
+    import ctypes as C
+    fn = orc.lib.orc_density_splat
+    fn.restype = None
+    fn.argtypes = [C.c_int64] * 3 + [np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"), C.c_int64, C.c_float,
+                                     C.c_int]
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(n, 3)).astype(np.float32)
+    if n == 1:
+        pos[0] = [8.0, 8.0, 8.0]
+    else:
+        pos[: n // 10] = np.clip(np.round(pos[: n // 10] / 16) * 16 + rng.uniform(-0.6, 0.6, size=(n // 10, 3)),
+                                 0.0, np.array(s) - 1e-3)
+        pos = pos.astype(np.float32)
+    g, slab = density_grid(pkg, orc, s, pos, r_scan)
+    want = run(fn, g, slab, pos, r_scan)
+    dg = DensityGrid(*s, capacity_blocks=g.n_slots + 4, lib=hip)
+    k = dg.splat(pos, r_scan)
+    b2s, got = dg.to_host()
+    assert k == g.n_slots - 2
+    assert np.array_equal(b2s >= 2, g.b2s >= 2)
+    act = np.nonzero(g.b2s >= 2)[0]
+    assert np.array_equal(got[b2s[act]], want[g.b2s[act]])
+    if n == 1:
+        assert got[b2s[0], 8 | (8 << 4) | (8 << 8)] == 3072
