@@ -225,6 +225,12 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
 /* The matching G2P: d_out (n x 3) = sum over the support cells of w * vel. */
 int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64_t n, float *d_out);
 
+/* MacGrid::divergence -- fluid/mac.rs:137-171: ensures `active` (host bids)
+ * in dst and writes ((u(x) - u(x-1)) + (v(y) - v(y-1))) + (w(z) - w(z-1)) per
+ * cell, a face at index 0 on its axis reading 0.  vel[3] and dst share the
+ * domain.  Each cell is stored at its layout index (the reference's running
+ * index, mac.rs:155-166, coincides with it for whole 16^3 blocks). */
+int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *active, int64_t n);
 /* particles::splat -- particles/splat.rs:92-319 with D = Density (u16,
  * channel.rs:303-375): the active blocks of the particles (footprint r_scan,
  * active.rs:15-47) are ensured in the layout grid g and filled with 65535 in

@@ -672,3 +672,56 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// MacGrid::divergence -- fluid/mac.rs:137-171 (each cell at its layout index;
+// the reference's running index coincides for whole blocks).
+// ===========================================================================
+namespace msbg {
+namespace {
+
+__device__ __forceinline__ float voxel_at(const GridView &g, int x, int y, int z) {
+    const int64_t bid = (x >> 4) + (int64_t)(y >> 4) * g.nx + (int64_t)(z >> 4) * g.nx * g.ny;
+    return g.slab[((int64_t)g.b2s[bid] << 12) + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8))];
+}
+
+__global__ __launch_bounds__(256) void divergence_kernel(GridView u, GridView v, GridView w, GridView d,
+                                                         const int32_t *__restrict__ bids, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n * 4096) return;
+    const int64_t bid = bids[i >> 12];
+    const int vid = (int)(i & 4095);
+    const int slot = d.b2s[bid];
+    if (slot < 2) return;
+    const int x = (int)(bid % d.nx) * 16 + (vid & 15), y = (int)((bid / d.nx) % d.ny) * 16 + ((vid >> 4) & 15),
+              z = (int)(bid / ((int64_t)d.nx * d.ny)) * 16 + (vid >> 8);
+    if (x >= d.sx || y >= d.sy || z >= d.sz) return;
+    const float ux1 = voxel_at(u, x, y, z), ux0 = x == 0 ? 0.0f : voxel_at(u, x - 1, y, z);
+    const float vy1 = voxel_at(v, x, y, z), vy0 = y == 0 ? 0.0f : voxel_at(v, x, y - 1, z);
+    const float wz1 = voxel_at(w, x, y, z), wz0 = z == 0 ? 0.0f : voxel_at(w, x, y, z - 1);
+    d.slab[((int64_t)slot << 12) + vid] = ((ux1 - ux0) + (vy1 - vy0)) + (wz1 - wz0);
+}
+
+}  // namespace
+}  // namespace msbg
+
+extern "C" int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *active, int64_t n) {
+    MSBG_CHECK(vel && vel[0] && vel[1] && vel[2] && dst, MSBG_EINVAL, "divergence: null grid");
+    MSBG_CHECK(n >= 0 && (n == 0 || active), MSBG_EINVAL, "divergence: bad active list");
+    for (int a = 0; a < 3; ++a)
+        MSBG_CHECK(vel[a]->L == 4 && vel[a]->sx == dst->sx && vel[a]->sy == dst->sy && vel[a]->sz == dst->sz,
+                   MSBG_EINVAL, "divergence: grids must share dimensions and 16^3 blocks");
+    MSBG_CHECK(dst->L == 4, MSBG_EINVAL, "divergence: 16^3-block destination required");
+    if (!n) return MSBG_OK;
+    int rc = msbg_grid_ensure_blocks(dst, active, n);  // dst.ensure_blocks_parallel(active), mac.rs:142
+    if (rc) return rc;
+    int32_t *d_bids = (int32_t *)workspace(15, sizeof(int32_t) * n, &rc);
+    if (rc) return rc;
+    std::vector<int32_t> h(active, active + n);
+    MSBG_HIP(hipMemcpyAsync(d_bids, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream()));
+    divergence_kernel<<<(int)((n * 4096 + 255) / 256), 256, 0, stream()>>>(vel[0]->view(), vel[1]->view(),
+                                                                             vel[2]->view(), dst->view(), d_bids, n);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}

@@ -73,6 +73,7 @@ _SIGS = {
     "msbg_p2g_bspline": [i32, vp, vp, vp, vp, vp, i64, C.POINTER(C.c_int64)],
     "msbg_g2p_bspline": [i32, vp, vp, i64, vp],
     "msbg_density_splat": [vp, vp, i64, vp, i64, f32, C.POINTER(C.c_int64)],
+    "msbg_mac_divergence": [vp, vp, vp, i64],
     "msbg_poisson_new": [vp, vp, i64, PP],
     "msbg_poisson_free": [vp],
     "msbg_poisson_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],

@@ -58,6 +58,11 @@ class MacGrid:
         self.mass = [mk(), mk(), mk()]
         self.sx, self.sy, self.sz = sx, sy, sz
 
+    def divergence(self, dst: DeviceGrid, active):
+        """MacGrid::divergence (fluid/mac.rs:137-171) into dst over `active` bids."""
+        a = np.ascontiguousarray(active, np.int64)
+        self.lib.call("msbg_mac_divergence", _handles(self.velocity), dst.h, a.ctypes.data, a.size)
+
 
 def particles_to_grid(mac: MacGrid, pos, vel, mass, r_p) -> int:
     """fluid/p2g.rs:141-164; returns the number of active blocks."""

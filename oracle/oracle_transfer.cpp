@@ -547,3 +547,40 @@ extern "C" void orc_density_splat(i64 sx, i64 sy, i64 sz, const i32 *b2s, uint16
         k0 = k1;
     }
 }
+
+// ===========================================================================
+// MacGrid::divergence -- fluid/mac.rs:137-171: for the cells of the `active`
+// blocks of dst, ((u(x) - u(x-1)) + (v(y) - v(y-1))) + (w(z) - w(z-1)), a face
+// at index 0 on its axis reading 0; velocity voxels read through each
+// channel's own bid -> slot table (absent / full blocks give the dummies).
+// The reference stores cell k of a block's clipped extent at vid = k (a
+// running index, mac.rs:155-166), which is the cell's layout index only for
+// whole 16^3 blocks; ref_packed_vid = 1 reproduces that, 0 stores every cell
+// at its layout index x | y << 4 | z << 8.
+// ===========================================================================
+extern "C" void orc_mac_divergence(i64 sx, i64 sy, i64 sz, const float *u, const i32 *ub2s, const float *v,
+                                   const i32 *vb2s, const float *w, const i32 *wb2s, float *dst, const i32 *db2s,
+                                   const i64 *active, i64 n, int ref_packed_vid) {
+    const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nxy = nx * ny;
+    auto at = [&](const float *s, const i32 *b2s, i64 x, i64 y, i64 z) {
+        return s[(i64)b2s[(x >> 4) + (y >> 4) * nx + (z >> 4) * nxy] * 4096 + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8))];
+    };
+    for (i64 k = 0; k < n; ++k) {
+        const i64 bid = active[k];
+        const i32 slot = db2s[bid];
+        if (slot < 2) continue;
+        const i64 gx0 = (bid % nx) * 16, gy0 = ((bid / nx) % ny) * 16, gz0 = (bid / nxy) * 16;
+        const i64 gx1 = std::min(gx0 + 16, sx), gy1 = std::min(gy0 + 16, sy), gz1 = std::min(gz0 + 16, sz);
+        i64 run = 0;
+        for (i64 z = gz0; z < gz1; ++z)
+            for (i64 y = gy0; y < gy1; ++y)
+                for (i64 x = gx0; x < gx1; ++x) {
+                    const float ux1 = at(u, ub2s, x, y, z), ux0 = x == 0 ? 0.0f : at(u, ub2s, x - 1, y, z);
+                    const float vy1 = at(v, vb2s, x, y, z), vy0 = y == 0 ? 0.0f : at(v, vb2s, x, y - 1, z);
+                    const float wz1 = at(w, wb2s, x, y, z), wz0 = z == 0 ? 0.0f : at(w, wb2s, x, y, z - 1);
+                    const i64 vid = ref_packed_vid ? run : ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8));
+                    dst[(i64)slot * 4096 + vid] = ((ux1 - ux0) + (vy1 - vy0)) + (wz1 - wz0);
+                    ++run;
+                }
+    }
+}

@@ -163,3 +163,23 @@ def test_density_splat_bitexact(pkg, orc, hip, r_scan, s, n):
     assert np.array_equal(got[b2s[act]], want[g.b2s[act]])
     if n == 1:
         assert got[b2s[0], 8 | (8 << 4) | (8 << 8)] == 3072
+
+
+@pytest.mark.parametrize("s,frac", [((32, 32, 32), 1.0), ((40, 33, 48), 0.7), ((64, 48, 80), 0.5)])
+def test_mac_divergence_bitexact(pkg, orc, hip, s, frac):
+    from test_oracle_transfer import mac_channels, oracle_divergence
+    f = lambda a, X, Y, Z: np.sin(0.37 * X + 0.2 * a) * np.cos(0.11 * Y + a) + 0.05 * Z * (a + 1)
+    chans = mac_channels(pkg, s, f, frac, seed=int(frac * 10))
+    rng = np.random.default_rng(1)
+    act = np.sort(rng.choice(chans[0].n_blocks, size=max(1, chans[0].n_blocks * 2 // 3), replace=False))
+    want = oracle_divergence(pkg, orc, chans, act)
+    mac = T.MacGrid(*s, capacity_blocks=chans[0].n_blocks + 4, lib=hip)
+    for a in range(3):
+        hip.call("msbg_grid_load", mac.velocity[a].h, chans[a].b2s.ctypes.data,
+                 np.ascontiguousarray(chans[a].slab).ctypes.data, chans[a].n_slots)
+    dst = pkg.DeviceGrid(*s, 4, 0.0, 0.0, capacity_blocks=chans[0].n_blocks + 4, lib=hip)
+    mac.divergence(dst, act)
+    got = dst.to_host()
+    assert np.array_equal(got.b2s >= 2, want.b2s >= 2)
+    for b in act:
+        assert_bitexact(got.slab[got.b2s[b]], want.slab[want.b2s[b]], "div block %d" % b)

@@ -177,3 +177,59 @@ def test_bspline_g2p_reproduces_affine_field(orc, order):
     want = np.stack([0.5 * p[:, 0] - 0.25 * p[:, 1] + 2.0, 0.1 * p[:, 2] + 1.0,
                      -0.3 * p[:, 0] + 0.2 * p[:, 1] + 0.1 * p[:, 2]], 1)
     np.testing.assert_allclose(out, want, atol=2e-5)
+
+
+def mac_channels(pkg, s, f, frac=1.0, seed=0):
+    """Three velocity channels (each its own grid) filled with f(axis, X, Y, Z)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for a in range(3):
+        g = pkg.SparseGrid("u%d" % a, *s, 0.0, 0.0)
+        bids = np.nonzero(rng.random(g.n_blocks) < frac)[0] if frac < 1 else np.arange(g.n_blocks)
+        g.ensure_blocks_parallel(bids)
+        g.fill_from_function(bids, lambda X, Y, Z, a=a: f(a, X, Y, Z).astype(np.float32))
+        out.append(g)
+    return out
+
+
+def oracle_divergence(pkg, orc, chans, active, ref_packed=0):
+    s = (chans[0].sx, chans[0].sy, chans[0].sz)
+    d = pkg.SparseGrid("div", *s, 0.0, 0.0)
+    d.ensure_blocks_parallel(active)
+    slab = np.zeros((d.n_slots, 4096), np.float32)
+    u, v, w = (np.ascontiguousarray(c.slab) for c in chans)
+    orc.orc_mac_divergence(*s, u, chans[0].b2s, v, chans[1].b2s, w, chans[2].b2s, slab, d.b2s,
+                           np.ascontiguousarray(active, np.int64), len(active), ref_packed)
+    d._slab[: d.n_slots] = slab
+    return d
+
+
+def test_div_10_uniform_and_linear(pkg, orc):
+    """fluid/mac.rs:193-225: u(i) = i + 1, v(j) = j + 1, w(k) = k + 1 -> 3 everywhere
+    (the index-0 faces read 0)."""
+    chans = mac_channels(pkg, (32, 32, 32), lambda a, X, Y, Z: (X, Y, Z)[a] + 1.0)
+    d = oracle_divergence(pkg, orc, chans, np.arange(8))
+    assert np.all(d.to_dense() == 3.0)
+
+
+def test_divergence_layout_index_and_reference_packing(pkg, orc):
+    """Whole blocks: the reference's running index is the layout index.  A
+    partial block (domain 40 = 2.5 blocks): the reference packs the clipped
+    extent, the layout index puts every cell where get_voxel reads it."""
+    f = lambda a, X, Y, Z: np.sin(0.3 * X + 0.2 * a) * np.cos(0.1 * Y) + 0.05 * Z * (a + 1)
+    chans = mac_channels(pkg, (40, 32, 32), f)
+    act = np.arange(chans[0].n_blocks)
+    exact = oracle_divergence(pkg, orc, chans, act, 0)
+    packed = oracle_divergence(pkg, orc, chans, act, 1)
+    D = exact.to_dense()
+    X = [c.to_dense().astype(np.float64) for c in chans]
+    want = np.zeros_like(D, np.float64)
+    for a, ax in ((0, 2), (1, 1), (2, 0)):
+        lo = np.concatenate([np.zeros_like(np.take(X[a], [0], axis=ax)), np.take(X[a], range(X[a].shape[ax] - 1),
+                                                                                  axis=ax)], axis=ax)
+        want = want + (X[a] - lo)
+    assert np.allclose(D, want, rtol=0, atol=1e-5)
+    whole = [b for b in act if (b % 3) < 2]  # blocks 0, 1 along x are whole, block 2 holds x 32..39
+    assert np.array_equal(exact.slab[exact.b2s[whole]], packed.slab[packed.b2s[whole]])
+    part = [b for b in act if (b % 3) == 2]
+    assert not np.array_equal(exact.slab[exact.b2s[part]], packed.slab[packed.b2s[part]])
