@@ -121,6 +121,19 @@ int msbg_sweep_plan_run_graph_async(msbg_sweep_plan *p, int stencil, float dt, i
 int msbg_sweep_plan_run_timed(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01,
                               float *kernel_ms, int64_t *launches);
 
+/* Sweeper<Density> -- solver.rs:154-240 with D = Density (u16): the same
+ * sweeps over a u16 slab d_slab16 laid out by `layout`'s bid -> slot table
+ * (slots 0 / 1 hold the empty / full values).  The halo is dequantised as
+ * u * (1/65535) (Density::to_f32, channel.rs:124-128) and every result is
+ * stored as round(v * 65535), half away from zero (StoreBack<Density>,
+ * math/stencil.rs:162-174), saturated to [0, 65535]; use clamp01 = 1 for
+ * the [0, 1] range the reference's quantiser assumes.  Single level. */
+int msbg_sweep_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *active, int64_t n, int stencil, float dt,
+                       int iterations, int clamp01);
+/* A plan over a u16 slab (run with msbg_sweep_plan_run_async / _run_graph_async). */
+int msbg_sweep_plan_new_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *active, int64_t n,
+                                msbg_sweep_plan **out);
+
 /* Halo topology -- the neighbour resolution of halo.rs:59-80 (single level,
  * coarse = NULL) or halo.rs:234-251 (multires): for each bid a row of
  * (2H+1)^3 int32 codes, index dx + n*dy + n*n*dz, code >= 0 = the neighbour's

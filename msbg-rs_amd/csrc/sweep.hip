@@ -28,6 +28,24 @@ constexpr int kSweepThreads = 512;
 // it is written as four aligned float4; x-halo cells at [4-H, 4), [20, 20+H).
 constexpr int kPitch = 24;
 
+// Element storage of a fine slab: FP32, or Density (u16, channel.rs:118-142)
+// loaded as u * (1/65535) (Density::to_f32, the halo's dequant, halo.rs:95)
+// and stored as round(v * 65535), half away from zero (StoreBack<Density>,
+// math/stencil.rs:162-174), saturated to the u16 range.
+__device__ __forceinline__ float deq(float v) { return v; }
+__device__ __forceinline__ float deq(uint16_t v) { return (float)v * (1.0f / 65535.0f); }
+__device__ __forceinline__ float4 load4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ float4 load4(const uint16_t *p) {
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return make_float4(deq((uint16_t)(u.x & 0xffffu)), deq((uint16_t)(u.x >> 16)), deq((uint16_t)(u.y & 0xffffu)),
+                       deq((uint16_t)(u.y >> 16)));
+}
+__device__ __forceinline__ void store_back(float *p, float v) { *p = v; }
+__device__ __forceinline__ void store_back(uint16_t *p, float v) {
+    const float r = fminf(fmaxf(roundf(v * 65535.0f), 0.0f), 65535.0f);
+    *p = (uint16_t)(int)r;
+}
+
 // ---------------------------------------------------------------------------
 // Halo topology.  Entry t of a block's row (t = dx + n*dy + n*n*dz over the
 // (2H+1)^3 neighbourhood) is the neighbour's fine slot (>= 0; 0 = empty dummy,
@@ -75,14 +93,16 @@ __device__ __forceinline__ void resolve_neighbours(const GridView &g, const Grid
 // boundary condition; a coarse neighbour is sampled at (x>>1, y>>1, z>>1) in
 // its 8^3 block.
 // ---------------------------------------------------------------------------
-template <int HALO, bool MULTI>
-__device__ __forceinline__ const float *cell_ptr(const GridView &g, const GridView &c, const int *s_nb,
-                                                 const int *s_nbc, int bx0, int by0, int bz0, int rx, int ry, int rz) {
+// (The coarse level is FP32 only: a coarse pointer is returned as TS = float.)
+template <int HALO, bool MULTI, typename TS>
+__device__ __forceinline__ const TS *cell_ptr(const TS *fs, const GridView &c, const int *s_nb, const int *s_nbc,
+                                              int bx0, int by0, int bz0, int rx, int ry, int rz) {
     constexpr int n = 2 * HALO + 1;
     const int nb = ((rx >> 4) - bx0 + HALO) + n * ((ry >> 4) - by0 + HALO) + n * n * ((rz >> 4) - bz0 + HALO);
     if (MULTI && s_nbc[nb] >= 0)
-        return c.slab + (int64_t)s_nbc[nb] * 512 + (((rx >> 1) & 7) | (((ry >> 1) & 7) << 3) | (((rz >> 1) & 7) << 6));
-    return g.slab + (int64_t)s_nb[nb] * 4096 + ((rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8));
+        return reinterpret_cast<const TS *>(c.slab + (int64_t)s_nbc[nb] * 512 +
+                                            (((rx >> 1) & 7) | (((ry >> 1) & 7) << 3) | (((rz >> 1) & 7) << 6)));
+    return fs + (int64_t)s_nb[nb] * 4096 + ((rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8));
 }
 
 // Gather tile, dense HSX^3 layout.  Every lane resolves all its cells first
@@ -111,7 +131,7 @@ __device__ __forceinline__ void fill_tile_dense(float *tile, const GridView &g, 
         int rx, ry, rz;
         if (resolve_axis(x0 + hx - HALO, g.sx - 1, bc, rx) && resolve_axis(y0 + hy - HALO, g.sy - 1, bc, ry) &&
             resolve_axis(z0 + hz - HALO, g.sz - 1, bc, rz))
-            src[k] = cell_ptr<HALO, MULTI>(g, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
+            src[k] = cell_ptr<HALO, MULTI, float>(g.slab, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
         // else Dirichlet outside the domain: the empty sentinel
     }
     float v[ITEMS];
@@ -132,9 +152,10 @@ __device__ __forceinline__ int tile_at(int hx, int hy, int hz) {
 
 // Pitched per-cell fill: the path for a block whose rows cross the domain's
 // x end (partial last block, halo.rs:130-144).
-template <int HALO, bool MULTI>
-__device__ void fill_tile_pitched_generic(float *tile, const GridView &g, const GridView &c, const int *s_nb,
-                                          const int *s_nbc, int bx0, int by0, int bz0, float empty_f32) {
+template <int HALO, bool MULTI, typename TS>
+__device__ void fill_tile_pitched_generic(float *tile, const TS *fs, const GridView &g, const GridView &c,
+                                          const int *s_nb, const int *s_nbc, int bx0, int by0, int bz0,
+                                          float empty_f32) {
     constexpr int B = 16, HSX = B + 2 * HALO, NT = HSX * HSX * HSX;
     const int x0 = bx0 * B, y0 = by0 * B, z0 = bz0 * B;
     for (int i = threadIdx.x; i < NT; i += blockDim.x) {
@@ -144,7 +165,7 @@ __device__ void fill_tile_pitched_generic(float *tile, const GridView &g, const 
         if (resolve_axis(x0 + hx - HALO, g.sx - 1, MSBG_BC_NEUMANN, rx) &&
             resolve_axis(y0 + hy - HALO, g.sy - 1, MSBG_BC_NEUMANN, ry) &&
             resolve_axis(z0 + hz - HALO, g.sz - 1, MSBG_BC_NEUMANN, rz))
-            v = *cell_ptr<HALO, MULTI>(g, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
+            v = deq(*cell_ptr<HALO, MULTI, TS>(fs, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz));
         tile[tile_at<HALO>(hx, hy, hz)] = v;
     }
 }
@@ -154,16 +175,16 @@ __device__ void fill_tile_pitched_generic(float *tile, const GridView &g, const 
 // (four float4 loads, or two float2 loads duplicated for a coarse
 // neighbour), its 2H x-halo cells are loaded one by one.  `skip_edges` leaves
 // out the rows and x-halo cells a 7-point stencil never reads.
-template <int HALO, bool MULTI, int NTHR>
-__device__ __forceinline__ void fill_tile_rows(float *tile, const GridView &g, const GridView &c, const int *s_nb,
-                                               const int *s_nbc, int bx0, int by0, int bz0, float empty_f32,
-                                               bool skip_edges) {
+template <int HALO, bool MULTI, int NTHR, typename TS>
+__device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const GridView &g, const GridView &c,
+                                               const int *s_nb, const int *s_nbc, int bx0, int by0, int bz0,
+                                               float empty_f32, bool skip_edges) {
     constexpr int B = 16, L = 4, HSX = B + 2 * HALO, n = 2 * HALO + 1;
     constexpr int NR = HSX * HSX, NQ = 4 * NR, NH = 2 * HALO * NR;
     constexpr int QI = (NQ + NTHR - 1) / NTHR, HI = (NH + NTHR - 1) / NTHR;
     const int x0 = bx0 * B, y0 = by0 * B, z0 = bz0 * B;
     const int t = threadIdx.x;
-    const float *qp[QI];
+    const TS *qp[QI];
     uint32_t qcoarse = 0, qwrite = 0;
 #pragma unroll
     for (int k = 0; k < QI; ++k) {  // interior quads: item = row * 4 + q
@@ -182,12 +203,13 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const GridView &g, c
         const int nb = HALO + n * ((ry >> L) - by0 + HALO) + n * n * ((rz >> L) - bz0 + HALO);
         if (MULTI && s_nbc[nb] >= 0) {
             qcoarse |= 1u << k;
-            qp[k] = c.slab + (int64_t)s_nbc[nb] * 512 + (((ry >> 1) & 7) << 3) + (((rz >> 1) & 7) << 6) + 2 * q;
+            qp[k] = reinterpret_cast<const TS *>(c.slab + (int64_t)s_nbc[nb] * 512 + (((ry >> 1) & 7) << 3) +
+                                                 (((rz >> 1) & 7) << 6) + 2 * q);
         } else {
-            qp[k] = g.slab + (int64_t)s_nb[nb] * 4096 + ((ry & 15) << 4) + ((rz & 15) << 8) + 4 * q;
+            qp[k] = fs + (int64_t)s_nb[nb] * 4096 + ((ry & 15) << 4) + ((rz & 15) << 8) + 4 * q;
         }
     }
-    const float *hp[HI];
+    const TS *hp[HI];
     uint32_t hwrite = 0;
 #pragma unroll
     for (int k = 0; k < HI; ++k) {  // x-halo cells: item = row * 2H + j; j < H left
@@ -204,7 +226,7 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const GridView &g, c
         if (resolve_axis(gx, g.sx - 1, MSBG_BC_NEUMANN, rx) &&
             resolve_axis(y0 + hy - HALO, g.sy - 1, MSBG_BC_NEUMANN, ry) &&
             resolve_axis(z0 + hz - HALO, g.sz - 1, MSBG_BC_NEUMANN, rz))
-            hp[k] = cell_ptr<HALO, MULTI>(g, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
+            hp[k] = cell_ptr<HALO, MULTI, TS>(fs, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
     }
     float4 qv[QI];
 #pragma unroll
@@ -215,12 +237,12 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const GridView &g, c
             const float2 c2 = *reinterpret_cast<const float2 *>(qp[k]);
             qv[k] = make_float4(c2.x, c2.x, c2.y, c2.y);
         } else {
-            qv[k] = *reinterpret_cast<const float4 *>(qp[k]);
+            qv[k] = load4(qp[k]);
         }
     }
     float hv[HI];
 #pragma unroll
-    for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? *hp[k] : empty_f32;
+    for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? deq(*hp[k]) : empty_f32;
 #pragma unroll
     for (int k = 0; k < QI; ++k)
         if ((qwrite >> k) & 1u) {
@@ -246,14 +268,14 @@ struct Cross {
 // z-halves; each lane walks 8 cells up its z column keeping the planes it
 // still needs in registers (5 / 9 / 13 LDS reads per cell for the 7 / 19 /
 // 25-tap stencils once the window is primed).
-template <int STENCIL, bool MULTI>
-__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridView c, float *__restrict__ fslab,
+template <int STENCIL, bool MULTI, typename TS>
+__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridView c, TS *__restrict__ fslab,
                                                               const float *__restrict__ cslab, const int64_t *bids,
                                                               const int32_t *topo, float dt, int clamp01,
                                                               float empty_f32) {
+    static_assert(!MULTI || sizeof(TS) == sizeof(float), "cross-level sweeps are FP32 only");
     // The slabs come in as plain pointer arguments so every access compiles
     // to a global (not flat) memory instruction.
-    g.slab = fslab;
     c.slab = const_cast<float *>(cslab);
     constexpr int HALO = STENCIL == MSBG_BILAPLACIAN ? 2 : 1;
     constexpr int HSX = 16 + 2 * HALO, NN = 2 * HALO + 1, NB = NN * NN * NN, CENTER = (NB - 1) / 2;
@@ -274,14 +296,14 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridVi
     const int bx0 = (int)(bid % g.nx), by0 = (int)((bid / g.nx) % g.ny), bz0 = (int)(bid / ((int64_t)g.nx * g.ny));
     __syncthreads();
     if (bx0 * 16 + 16 <= g.sx)
-        fill_tile_rows<HALO, MULTI, kSweepThreads>(tile, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32,
+        fill_tile_rows<HALO, MULTI, kSweepThreads, TS>(tile, fslab, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32,
                                                    STENCIL == MSBG_LAPLACIAN);
     else
-        fill_tile_pitched_generic<HALO, MULTI>(tile, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32);
+        fill_tile_pitched_generic<HALO, MULTI, TS>(tile, fslab, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32);
     __syncthreads();
 
     const int x = threadIdx.x & 15, y = (threadIdx.x >> 4) & 15, zs = (threadIdx.x >> 8) * CPT;
-    float *out = g.slab + (int64_t)slot * 4096 + y * 16 + x;
+    TS *out = fslab + (int64_t)slot * 4096 + y * 16 + x;
     const bool cl = clamp01 != 0;
     // T(dx, dy, z): tile value at block cell (x+dx, y+dy, z).
     auto T = [&](int dx, int dy, int z) -> float {
@@ -303,7 +325,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridVi
             const float fx = (rx - lx) * 0.5f, fy = (dyd - dyu) * 0.5f, fz = (dzf - dzb) * 0.5f;
             const float gm = __builtin_sqrtf(__builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz)));
             const float v = __builtin_fmaf(dt * lap, gm, cc);
-            out[z * 256] = cl ? clamp_f(v, 0.0f, 1.0f) : v;
+            store_back(out + z * 256, cl ? clamp_f(v, 0.0f, 1.0f) : v);
             cm = c0;
             c0 = cp;
         }
@@ -331,7 +353,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridVi
             const float hc = grad > 1e-7f ? hnum / grad : 0.0f;
             const float d = clamp_f(dt * hc, -0.1f, 0.1f);
             const float v = f0 + d;
-            out[z * 256] = cl ? clamp_f(v, 0.0f, 1.0f) : v;
+            store_back(out + z * 256, cl ? clamp_f(v, 0.0f, 1.0f) : v);
             pm = p0;
             p0 = pp;
         }
@@ -354,7 +376,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(GridView g, GridVi
             const float upd = (dt * hh) * gm;
             const float d = clamp_f(-upd, -0.1f, 0.1f);
             const float v = f0 + d;
-            out[z * 256] = cl ? clamp_f(v, 0.0f, 1.0f) : v;
+            store_back(out + z * 256, cl ? clamp_f(v, 0.0f, 1.0f) : v);
             cm2 = pm.c;
             pm = p0;
             p0 = pp;
@@ -394,6 +416,7 @@ struct msbg_sweep_plan {
     int32_t *d_topo[3] = {nullptr, nullptr, nullptr};  // by halo width 1, 2
     int64_t counts[8] = {0};
     int64_t n = 0;
+    uint16_t *slab16 = nullptr;  // Density (u16) storage over g's layout, or null (FP32)
     // Cached graph of the last replayed sweep (run_graph).
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -506,6 +529,25 @@ int plan_new(msbg_grid *g, const msbg_grid *coarse, const uint8_t *d_levels, con
     return MSBG_OK;
 }
 
+template <bool MULTI, typename TS>
+void launch_colour(int stencil, dim3 grid, hipStream_t s, GridView g, GridView c, TS *fslab, const int64_t *bids,
+                   const int32_t *topo, float dt, int clamp01, float empty_f32) {
+    const dim3 block(kSweepThreads);
+    switch (stencil) {
+        case MSBG_LAPLACIAN:
+            hipLaunchKernelGGL((sweep_kernel<MSBG_LAPLACIAN, MULTI, TS>), grid, block, 0, s, g, c, fslab, c.slab, bids,
+                               topo, dt, clamp01, empty_f32);
+            break;
+        case MSBG_MEAN_CURVATURE:
+            hipLaunchKernelGGL((sweep_kernel<MSBG_MEAN_CURVATURE, MULTI, TS>), grid, block, 0, s, g, c, fslab, c.slab,
+                               bids, topo, dt, clamp01, empty_f32);
+            break;
+        default:
+            hipLaunchKernelGGL((sweep_kernel<MSBG_BILAPLACIAN, MULTI, TS>), grid, block, 0, s, g, c, fslab, c.slab,
+                               bids, topo, dt, clamp01, empty_f32);
+    }
+}
+
 // Launches the 8 colour passes of one sweep on `s`.  When `ev` is non-null,
 // an event pair brackets every launch.
 template <bool MULTI>
@@ -513,14 +555,14 @@ int launch_sweep(msbg_sweep_plan *p, hipStream_t s, int stencil, float dt, int c
                  std::vector<hipEvent_t> *ev = nullptr) {
     GridView g = p->g->view();
     GridView c = MULTI ? p->coarse->view() : g;
-    const float empty_f32 = MULTI ? 0.0f : p->g->empty_value;
+    const float empty_f32 = MULTI ? 0.0f : (p->slab16 ? 0.0f : p->g->empty_value);
     const int halo = stencil == MSBG_BILAPLACIAN ? 2 : 1;
     const int nb = halo == 2 ? 125 : 27;
     int64_t off = 0;
     for (int col = 0; col < 8; ++col) {
         const int64_t cnt = p->counts[col];
         if (cnt > 0) {
-            dim3 grid((unsigned)cnt), block(kSweepThreads);
+            const dim3 grid((unsigned)cnt);
             const int64_t *bids = p->d_bids + off;
             const int32_t *topo = p->d_topo[halo] + off * nb;
             if (ev) {
@@ -531,19 +573,10 @@ int launch_sweep(msbg_sweep_plan *p, hipStream_t s, int stencil, float dt, int c
                 ev->push_back(b);
                 MSBG_HIP(hipEventRecord(a, s));
             }
-            switch (stencil) {
-                case MSBG_LAPLACIAN:
-                    hipLaunchKernelGGL((sweep_kernel<MSBG_LAPLACIAN, MULTI>), grid, block, 0, s, g, c, g.slab, c.slab, bids, topo,
-                                       dt, clamp01, empty_f32);
-                    break;
-                case MSBG_MEAN_CURVATURE:
-                    hipLaunchKernelGGL((sweep_kernel<MSBG_MEAN_CURVATURE, MULTI>), grid, block, 0, s, g, c, g.slab, c.slab, bids,
-                                       topo, dt, clamp01, empty_f32);
-                    break;
-                default:
-                    hipLaunchKernelGGL((sweep_kernel<MSBG_BILAPLACIAN, MULTI>), grid, block, 0, s, g, c, g.slab, c.slab, bids, topo,
-                                       dt, clamp01, empty_f32);
-            }
+            if (!MULTI && p->slab16)
+                launch_colour<false, uint16_t>(stencil, grid, s, g, c, p->slab16, bids, topo, dt, clamp01, empty_f32);
+            else
+                launch_colour<MULTI, float>(stencil, grid, s, g, c, g.slab, bids, topo, dt, clamp01, empty_f32);
             MSBG_HIP(hipGetLastError());
             if (ev) MSBG_HIP(hipEventRecord(ev->back(), s));
         }
@@ -622,6 +655,31 @@ int msbg_color_buckets(const msbg_grid *g, const int64_t *active, int64_t n, int
 
 int msbg_sweep_plan_new(msbg_grid *g, const int64_t *active, int64_t n, msbg_sweep_plan **out) {
     return plan_new(g, nullptr, nullptr, active, n, out);
+}
+
+int msbg_sweep_plan_new_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *active, int64_t n,
+                                msbg_sweep_plan **out) {
+    MSBG_CHECK(d_slab16, MSBG_EINVAL, "density plan: null u16 slab");
+    int rc = plan_new(layout, nullptr, nullptr, active, n, out);
+    if (rc) return rc;
+    (*out)->slab16 = d_slab16;
+    return MSBG_OK;
+}
+
+int msbg_sweep_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *active, int64_t n, int stencil, float dt,
+                       int iterations, int clamp01) {
+    int rc = check_stencil(stencil);
+    if (rc) return rc;
+    msbg_sweep_plan *p = nullptr;
+    rc = msbg_sweep_plan_new_density(layout, d_slab16, active, n, &p);
+    if (rc) return rc;
+    rc = msbg_sweep_plan_run_async(p, stencil, dt, iterations, clamp01);
+    if (rc == MSBG_OK) {
+        hipError_t e = hipStreamSynchronize(stream());
+        if (e != hipSuccess) rc = hip_fail(e, "msbg_sweep_density");
+    }
+    msbg_sweep_plan_free(p);
+    return rc;
 }
 
 int msbg_sweep_plan_new_multires(msbg_grid *fine, const msbg_grid *coarse, const uint8_t *dev_levels,

@@ -49,6 +49,8 @@ _SIGS = {
     "msbg_sweep": [vp, vp, i64, i32, f32, i32, i32],
     "msbg_sweep_plan_new": [vp, vp, i64, PP],
     "msbg_sweep_plan_free": [vp],
+    "msbg_sweep_density": [vp, vp, vp, i64, i32, f32, i32, i32],
+    "msbg_sweep_plan_new_density": [vp, vp, vp, i64, PP],
     "msbg_sweep_plan_run_async": [vp, i32, f32, i32, i32],
     "msbg_sweep_plan_run_timed": [vp, i32, f32, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_int64)],
     "msbg_sweep_plan_run_graph_async": [vp, i32, f32, i32, i32],

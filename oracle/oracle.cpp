@@ -455,6 +455,53 @@ void orc_sweep(i64 sx, i64 sy, i64 sz, float *slab, const i32 *b2s, const i64 *a
     }
 }
 
+// Sweeper::sweep for D = Density (u16): the halo is filled from the
+// dequantised field u * (1/65535) (Density::to_f32, channel.rs:124-128;
+// halo.rs:95) and every smoothed value is stored as round(v * 65535), half
+// away from zero (StoreBack<Density>, math/stencil.rs:162-174), saturated to
+// [0, 65535].  A FP32 mirror of the dequantised slab carries the in-place
+// updates between colour passes.
+void orc_sweep_density(i64 sx, i64 sy, i64 sz, uint16_t *slab16, i64 n_slots, const i32 *b2s, const i64 *active,
+                       i64 n, int stencil, float dt, int iterations, int clamp01) {
+    const int L = 4, B = 16;
+    const float inv = 1.0f / 65535.0f;
+    std::vector<float> f((size_t)n_slots * 4096);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)slab16[i] * inv;
+    std::vector<i64> order(n > 0 ? n : 1);
+    i64 counts[8];
+    orc_color_buckets(sx, sy, sz, L, active, n, order.data(), counts);
+    const int halo = stencil == 2 ? 2 : 1;
+    const int full = stencil == 0 ? 0 : 1;
+    const int HSX = B + 2 * halo;
+    std::vector<float> tile((size_t)HSX * HSX * HSX), out(4096);
+    for (int it = 0; it < iterations; ++it) {
+        i64 off = 0;
+        for (int c = 0; c < 8; ++c) {
+            for (i64 k = 0; k < counts[c]; ++k) {
+                const i64 bid = order[off + k];
+                const i32 s = b2s[bid];
+                if (s < 2) continue;
+                orc_halo_fill(sx, sy, sz, L, f.data(), b2s, bid, halo, full, BC_NEUMANN, tile.data());
+                std::memcpy(out.data(), &f[(size_t)s * 4096], 4096 * sizeof(float));
+                if (stencil == 0)
+                    orc_kernel_laplacian(B, tile.data(), dt, clamp01, out.data());
+                else if (stencil == 1)
+                    orc_kernel_meancurv(B, tile.data(), dt, clamp01, out.data());
+                else
+                    orc_kernel_bilaplacian(B, tile.data(), dt, clamp01, out.data());
+                // cells past a partial domain end are written by the kernels too; store them all
+                for (int v = 0; v < 4096; ++v) {
+                    const float r = std::min(std::max(roundf(out[v] * 65535.0f), 0.0f), 65535.0f);
+                    const uint16_t q = (uint16_t)(int)r;
+                    slab16[(size_t)s * 4096 + v] = q;
+                    f[(size_t)s * 4096 + v] = (float)q * inv;
+                }
+            }
+            off += counts[c];
+        }
+    }
+}
+
 // Multires sweep: the same colour schedule over the level-l blocks in
 // `active`, with the halo gathered by fill_multires (halo.rs:187) from level l
 // and level l+1.  The reference runs fill_multires only from benches

@@ -76,6 +76,8 @@ def load():
     _sig(lib, "orc_cg64", i32, i64, i64, i64, f64p, f64p, i64, i32p, i64p, i64, C.c_double, i32)
     _sig(lib, "orc_mac_divergence", None, i64, i64, i64, f32p, i32p, f32p, i32p, f32p, i32p, f32p, i32p, i64p, i64,
          i32)
+    _sig(lib, "orc_sweep_density", None, i64, i64, i64, np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), i64,
+         i32p, i64p, i64, i32, f32, i32, i32)
     _sig(lib, "orc_net_param_count", i32)
     _sig(lib, "orc_net_forward", None, f32p, f32p, i64, i32, f32p)
     _sig(lib, "orc_net_loss_grad", C.c_double, f32p, f32p, f32p, i64, i32, f64p)

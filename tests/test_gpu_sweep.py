@@ -256,3 +256,28 @@ def test_graph_replay_matches_eager(pkg, orc, hip, stencil):
         assert_bitexact(gb.to_host().slab, ga.to_host().slab, "replay %d" % rep)
     orc.sweep(g, active, stencil, 0.02, 6, True)
     assert_bitexact(gb.to_host().slab, g.slab, "graph vs oracle")
+
+
+@pytest.mark.parametrize("stencil,dt", [(0, 0.1), (1, 0.02), (2, 0.005)])
+@pytest.mark.parametrize("s", [(40, 33, 48), (64, 48, 80)])
+def test_density_sweep_bitexact(pkg, orc, hip, stencil, dt, s):
+    """Sweeper<Density>: u16 storage, dequantised halo, round-half-away store;
+    eager and graph-replayed plans against the oracle, two iterations."""
+    import ctypes as C
+    from test_oracle_density import density_of, sweep16
+    g = random_grid(pkg, *s, seed=11 + stencil, lo=0.0, hi=1.0, empty=0.0, full=1.0)
+    active = np.nonzero(g.b2s >= 2)[0].astype(np.int64)
+    q = density_of(g)
+    q[0], q[1] = 0, 65535
+    want = sweep16(orc, g, q, active, stencil, dt, it=2)
+    layout = pkg.DeviceGrid.from_host(g, lib=hip)
+    d = hip.to_device(q)
+    hip.call("msbg_sweep_density", layout.h, d.ptr, active.ctypes.data, active.size, stencil, dt, 2, 1)
+    assert np.array_equal(d.get(), want)
+    d2 = hip.to_device(q)
+    plan = C.c_void_p()
+    hip.call("msbg_sweep_plan_new_density", layout.h, d2.ptr, active.ctypes.data, active.size, C.byref(plan))
+    hip.call("msbg_sweep_plan_run_graph_async", plan, stencil, dt, 2, 1)
+    hip.synchronize()
+    hip.call("msbg_sweep_plan_free", plan)
+    assert np.array_equal(d2.get(), want)
