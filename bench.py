@@ -216,7 +216,9 @@ def scale_up_leg(pkg, lib, S, steps):
     tmp = lib.empty(x.shape, np.float32)
     lib.call("msbg_memcpy_dtod", tmp.ptr, x.ptr, x.nbytes)
     zero = lib.empty(x.shape, np.float32)
-    lib.call("msbg_memcpy_htod", zero.ptr, np.zeros(x.shape, np.float32).ctypes.data, x.nbytes)
+    host_zero = np.zeros(1 << 24, np.float32)  # kept alive across the copies (64 MiB chunks)
+    for off in range(0, x.nbytes, host_zero.nbytes):
+        lib.call("msbg_memcpy_htod", zero.ptr + off, host_zero.ctypes.data, min(host_zero.nbytes, x.nbytes - off))
     band.jacobi_sweeps(0, x, tmp, zero, 2.0 / 3.0, 2)
     lib.synchronize()
     lib.call("msbg_event_record", e0)

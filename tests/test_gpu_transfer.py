@@ -174,9 +174,10 @@ def test_mac_divergence_bitexact(pkg, orc, hip, s, frac):
     act = np.sort(rng.choice(chans[0].n_blocks, size=max(1, chans[0].n_blocks * 2 // 3), replace=False))
     want = oracle_divergence(pkg, orc, chans, act)
     mac = T.MacGrid(*s, capacity_blocks=chans[0].n_blocks + 4, lib=hip)
+    slabs = [np.ascontiguousarray(c.slab) for c in chans]  # alive for the copies
     for a in range(3):
-        hip.call("msbg_grid_load", mac.velocity[a].h, chans[a].b2s.ctypes.data,
-                 np.ascontiguousarray(chans[a].slab).ctypes.data, chans[a].n_slots)
+        hip.call("msbg_grid_load", mac.velocity[a].h, chans[a].b2s.ctypes.data, slabs[a].ctypes.data,
+                 chans[a].n_slots)
     dst = pkg.DeviceGrid(*s, 4, 0.0, 0.0, capacity_blocks=chans[0].n_blocks + 4, lib=hip)
     mac.divergence(dst, act)
     got = dst.to_host()
