@@ -31,12 +31,26 @@ template <>
 struct Sto<MSBG_STORAGE_F32> {
     using T = float;
     static __device__ __forceinline__ float ld(const T *p, i64 i) { return p[i]; }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {  // i: multiple of 4
+        const float4 q = *reinterpret_cast<const float4 *>(p + i);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+    }
     static __device__ __forceinline__ T cv(float v) { return v; }
 };
 template <>
 struct Sto<MSBG_STORAGE_F16> {
     using T = uint16_t;
     static __device__ __forceinline__ float ld(const T *p, i64 i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
+        v[0] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.x & 0xffffu));
+        v[1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.x >> 16));
+        v[2] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.y & 0xffffu));
+        v[3] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.y >> 16));
+    }
     // The FP32 value is materialised before the RNE narrowing: without the
     // register barrier the compiler fuses fmaf + cvt into one mixed-precision
     // FMA (v_fma_mixlo_f16), which rounds once to f16 instead of f32 then f16.
@@ -49,6 +63,13 @@ template <>
 struct Sto<MSBG_STORAGE_BF16> {
     using T = uint16_t;
     static __device__ __forceinline__ float ld(const T *p, i64 i) { return __uint_as_float((uint32_t)p[i] << 16); }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
+        v[0] = __uint_as_float(q.x << 16);
+        v[1] = __uint_as_float(q.x & 0xffff0000u);
+        v[2] = __uint_as_float(q.y << 16);
+        v[3] = __uint_as_float(q.y & 0xffff0000u);
+    }
     static __device__ __forceinline__ T cv(float v) {
         uint32_t u = __float_as_uint(v);
         if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
@@ -86,6 +107,68 @@ __device__ __forceinline__ float residual_of(const typename Sto<ST>::T *x, const
     return __builtin_fmaf(-6.0f, xc, S::ld(b, k.c) + s);
 }
 
+// A quad: 4 x-adjacent cells (x = 4q .. 4q+3) of one row of a listed block
+// with their 6 neighbours, resolved exactly as cell_at: a neighbour outside
+// the domain is the cell itself, across a block face it is read from the
+// face-neighbour slot.  A work-group (256 threads) covers 4 z-slices of one
+// block: thread = quad q (0..3) x row y (16) x slice (4).
+struct Quad {
+    float c[4], l, r, ym[4], yp[4], zm[4], zp[4];
+    i64 c0;
+    int gx0, gy, gz, n;  // n = cells of the quad inside the domain (0 = none)
+};
+
+template <int ST>
+__device__ __forceinline__ void load_quad(const PBlock &B, const Dom &d, const typename Sto<ST>::T *x, int wg,
+                                          Quad &Q) {
+    using S = Sto<ST>;
+    const int t = threadIdx.x, q = t & 3, ty = (t >> 2) & 15, tz = ((wg % kSlices) << 2) | (t >> 6);
+    Q.gx0 = B.ox + 4 * q;
+    Q.gy = B.oy + ty;
+    Q.gz = B.oz + tz;
+    Q.n = 0;
+    if (Q.gx0 >= d.sx || Q.gy >= d.sy || Q.gz >= d.sz) return;
+    Q.n = min(4, d.sx - Q.gx0);
+    const int vid = 4 * q | ty << 4 | tz << 8;
+    const i64 base = (i64)B.slot << 12;
+    Q.c0 = base + vid;
+    S::ld4(x, Q.c0, Q.c);
+    Q.l = q > 0 ? S::ld(x, Q.c0 - 1) : (Q.gx0 == 0 ? Q.c[0] : S::ld(x, ((i64)B.nb[0] << 12) + (vid | 15)));
+    Q.r = Q.gx0 + 4 >= d.sx ? 0.0f : (q < 3 ? S::ld(x, Q.c0 + 4) : S::ld(x, ((i64)B.nb[1] << 12) + (vid & ~15)));
+    if (ty > 0)
+        S::ld4(x, Q.c0 - 16, Q.ym);
+    else if (Q.gy == 0)
+        for (int i = 0; i < 4; ++i) Q.ym[i] = Q.c[i];
+    else
+        S::ld4(x, ((i64)B.nb[2] << 12) + (vid | 0xf0), Q.ym);
+    if (Q.gy + 1 >= d.sy)
+        for (int i = 0; i < 4; ++i) Q.yp[i] = Q.c[i];
+    else if (ty < 15)
+        S::ld4(x, Q.c0 + 16, Q.yp);
+    else
+        S::ld4(x, ((i64)B.nb[3] << 12) + (vid & ~0xf0), Q.yp);
+    if (tz > 0)
+        S::ld4(x, Q.c0 - 256, Q.zm);
+    else if (Q.gz == 0)
+        for (int i = 0; i < 4; ++i) Q.zm[i] = Q.c[i];
+    else
+        S::ld4(x, ((i64)B.nb[4] << 12) + (vid | 0xf00), Q.zm);
+    if (Q.gz + 1 >= d.sz)
+        for (int i = 0; i < 4; ++i) Q.zp[i] = Q.c[i];
+    else if (tz < 15)
+        S::ld4(x, Q.c0 + 256, Q.zp);
+    else
+        S::ld4(x, ((i64)B.nb[5] << 12) + (vid & ~0xf00), Q.zp);
+}
+
+// t = fmaf(-6, x_c, b_c + s) of cell i of the quad, s summed -x, +x, -y, +y, -z, +z.
+__device__ __forceinline__ float quad_residual(const Quad &Q, const Dom &d, int i, float bi) {
+    const float xl = i == 0 ? Q.l : Q.c[i - 1];
+    const float xr = Q.gx0 + i + 1 >= d.sx ? Q.c[i] : (i == 3 ? Q.r : Q.c[i + 1]);
+    const float s = ((((xl + xr) + Q.ym[i]) + Q.yp[i]) + Q.zm[i]) + Q.zp[i];
+    return __builtin_fmaf(-6.0f, Q.c[i], bi + s);
+}
+
 // RB: in place, cells with (gx + gy + gz) & 1 == color (their neighbours all
 // have the other colour, so a launch never reads a cell it writes).
 // !RB: weighted Jacobi, x_in -> x_out (distinct buffers).
@@ -93,20 +176,20 @@ template <int ST, bool RB>
 __global__ __launch_bounds__(kThreads) void relax_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                          const typename Sto<ST>::T *xin, typename Sto<ST>::T *xout,
                                                          const typename Sto<ST>::T *b, float w6, int color) {
-    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int gx = B.ox + tx, gy = B.oy + ty;
-    if (gx >= d.sx || gy >= d.sy) return;
-    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / kSlices];
+    Quad Q;
+    load_quad<ST>(B, d, xin, wg, Q);
+    if (!Q.n) return;
+    float bq[4];
+    Sto<ST>::ld4(b, Q.c0, bq);
+    const int par = (Q.gx0 + Q.gy + Q.gz) & 1;
 #pragma unroll
-    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
-        const int gz = B.oz + tz;
-        if (gz >= d.sz) break;
-        if (RB && ((gx + gy + gz) & 1) != color) continue;
-        const Cell k = cell_at(B, d, tx, ty, tz, gx, gy, gz);
-        float xc;
-        const float t = residual_of<ST>(xin, b, k, xc);
-        xout[k.c] = Sto<ST>::cv(__builtin_fmaf(w6, t, xc));
+    for (int i = 0; i < 4; ++i) {
+        if (i >= Q.n) break;
+        if (RB && ((par + i) & 1) != color) continue;
+        const float t = quad_residual(Q, d, i, bq[i]);
+        xout[Q.c0 + i] = Sto<ST>::cv(__builtin_fmaf(w6, t, Q.c[i]));
     }
 }
 
@@ -117,22 +200,21 @@ __global__ __launch_bounds__(kThreads) void residual_kernel(const PBlock *__rest
                                                             const typename Sto<ST>::T *x,
                                                             const typename Sto<ST>::T *b, float *r,
                                                             double2 *partial) {
-    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int gx = B.ox + tx, gy = B.oy + ty;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / kSlices];
+    Quad Q;
+    load_quad<ST>(B, d, x, wg, Q);
     double rr = 0.0, bb = 0.0;
-    if (gx < d.sx && gy < d.sy) {
-        const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
-        for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
-            const int gz = B.oz + tz;
-            if (gz >= d.sz) break;
-            const Cell k = cell_at(B, d, tx, ty, tz, gx, gy, gz);
-            float xc;
-            const float t = residual_of<ST>(x, b, k, xc);
-            if (r) r[k.c] = t;
-            const double bv = Sto<ST>::ld(b, k.c);
+    if (Q.n) {
+        float bq[4];
+        Sto<ST>::ld4(b, Q.c0, bq);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= Q.n) break;
+            const float t = quad_residual(Q, d, i, bq[i]);
+            if (r) r[Q.c0 + i] = t;
             rr += (double)t * (double)t;
-            bb += bv * bv;
+            bb += (double)bq[i] * (double)bq[i];
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
