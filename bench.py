@@ -237,6 +237,46 @@ def scale_up_leg(pkg, lib, S, steps):
     return out
 
 
+def solve_scale_leg(pkg, lib, S, reps):
+    """Time to 1e-6 on a band sized past the Infinity Cache (the reference's
+    shell generator at S^3): FP32 sweeps vs FP16 / BF16 storage with FP32
+    refinement, where the half-width storage halves the streamed bytes."""
+    from msbg_rs_amd import scenes
+    nb = S // 16
+    bids = scenes.reference_shell_blocks(nb, nb, nb)
+    g = pkg.DeviceGrid(S, S, S, 4, 0.0, 1.0, capacity_blocks=bids.size + 8, lib=lib)
+    g.ensure_blocks_parallel(bids)
+    band = pkg.PoissonBand(g, bids, lib=lib)
+    # Right-hand side: LCG-uniform in [-1, 1] per listed block (as
+    # scenes.poisson_rhs, one stream per chunk of 4096 blocks), written into
+    # the layout grid's own slab and copied out as the b field.
+    for k in range(0, bids.size, 4096):
+        m = min(4096, bids.size - k)
+        u = scenes.lcg_uniform(11 + k, m * 4096) * np.float32(2.0) - np.float32(1.0)
+        g.write_blocks(bids[k:k + m], u)
+    b = lib.empty(band.n_slots * 4096, np.float32)
+    lib.call("msbg_memcpy_dtod", b.ptr, g.views()[0], band.n_slots * 4096 * 4)
+    out = {"domain": S, "blocks": int(bids.size), "cells": band.n_cells, "fp32_field_bytes": band.n_cells * 4}
+    for name, storage, refine in (("f32_plain", 0, False), ("f16_refine", 1, True), ("bf16_refine", 2, True)):
+        best = None
+        for _ in range(reps):
+            x = band.field(0)
+            st = band.solve(x, b, storage=storage, refine=refine, omega=1.9, sweeps_per_check=10, max_sweeps=20000,
+                            tol=1e-6)
+            x.free()
+            if best is None or st["ms"] < best["ms"]:
+                best = st
+        out[name] = {k: best[k] for k in ("ms", "sweeps", "rel_residual", "converged")}
+        out[name]["ms_per_sweep"] = best["ms"] / max(best["sweeps"], 1)
+    base = out["f32_plain"]["ms"]
+    for k in ("f32_plain", "f16_refine", "bf16_refine"):
+        out[k]["time_vs_f32_plain"] = out[k]["ms"] / base
+    b.free()
+    band.close()
+    g.free()
+    return out
+
+
 SOLVE_VARIANTS = (("f32_plain", 0, False, False), ("f32_learned", 0, False, True), ("f16_refine", 1, True, False),
                   ("bf16_refine", 2, True, False), ("f16_refine_learned", 1, True, True),
                   ("bf16_refine_learned", 2, True, True))
@@ -314,6 +354,8 @@ def main():
                     help="first run a plain FP32 copy kernel over MB MiB three times (counter calibration)")
     ap.add_argument("--scale-up", type=int, default=1536, metavar="S",
                     help="domain edge of the HBM-bound scale-up sweep leg (0 = skip)")
+    ap.add_argument("--solve-scale", type=int, default=1024, metavar="S",
+                    help="domain edge of the HBM-bound Poisson solve leg (0 = skip)")
     ap.add_argument("--fp64-band", action="store_true",
                     help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
     args = ap.parse_args()
@@ -425,6 +467,9 @@ def main():
     if args.scale_up > 0:
         log("scale-up leg ...")
         out["scale_up"] = scale_up_leg(pkg, lib, args.scale_up, 3)
+    if args.solve_scale > 0:
+        log("HBM-bound solve leg ...")
+        out["solve_scale"] = solve_scale_leg(pkg, lib, args.solve_scale, 1)
     if args.particles > 0:
         log("particle leg ...")
         out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)

@@ -18,7 +18,8 @@ struct Dom {
     int32_t sx, sy, sz;
 };
 
-// A relaxation work-group covers 4 z-slices of one block: 256 threads = 16 x 16 columns.
+// Per-block kernels of 256 threads; the cell-wise ones (correction, dots) take
+// kSlices work-groups per block, 4 z-slices each, 16 x 16 threads.
 constexpr int kThreads = 256;
 constexpr int kSlices = 4;
 
@@ -28,7 +29,8 @@ struct msbg_poisson {
     msbg::Dom dom{};
     int64_t n_slots = 0, n_active = 0, n_cells = 0;
     int64_t nx = 0, ny = 0, nz = 0, n_blocks = 0;
-    int n_wg = 0;
+    int n_wg = 0;                     // work-groups of the kSlices-per-block kernels
+    int zcol[3] = {1, 1, 1};          // column depth of the relaxation kernels, per storage
     msbg::PBlock *d_blocks = nullptr;
     int32_t *d_b2s = nullptr;        // copy of the layout grid's bid -> slot table
     uint8_t *d_listed = nullptr;     // per block: 1 if listed

@@ -17,6 +17,7 @@
 //   t = fmaf(-6, x_c, b_c + s)            (= (b - A x)_c)
 //   x_c' = fmaf(omega / 6, t, x_c)        rounded to the storage type (RNE)
 #include <cmath>
+#include <type_traits>
 
 #include "poisson.hpp"
 
@@ -44,12 +45,16 @@ template <>
 struct Sto<MSBG_STORAGE_F16> {
     using T = uint16_t;
     static __device__ __forceinline__ float ld(const T *p, i64 i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
+    static __device__ __forceinline__ float lo(uint32_t w) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    }
+    static __device__ __forceinline__ float hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
     static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
         const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
-        v[0] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.x & 0xffffu));
-        v[1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.x >> 16));
-        v[2] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.y & 0xffffu));
-        v[3] = (float)__builtin_bit_cast(_Float16, (uint16_t)(q.y >> 16));
+        v[0] = lo(q.x);
+        v[1] = hi(q.x);
+        v[2] = lo(q.y);
+        v[3] = hi(q.y);
     }
     // The FP32 value is materialised before the RNE narrowing: without the
     // register barrier the compiler fuses fmaf + cvt into one mixed-precision
@@ -63,12 +68,14 @@ template <>
 struct Sto<MSBG_STORAGE_BF16> {
     using T = uint16_t;
     static __device__ __forceinline__ float ld(const T *p, i64 i) { return __uint_as_float((uint32_t)p[i] << 16); }
+    static __device__ __forceinline__ float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    static __device__ __forceinline__ float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
     static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
         const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
-        v[0] = __uint_as_float(q.x << 16);
-        v[1] = __uint_as_float(q.x & 0xffff0000u);
-        v[2] = __uint_as_float(q.y << 16);
-        v[3] = __uint_as_float(q.y & 0xffff0000u);
+        v[0] = lo(q.x);
+        v[1] = hi(q.x);
+        v[2] = lo(q.y);
+        v[3] = hi(q.y);
     }
     static __device__ __forceinline__ T cv(float v) {
         uint32_t u = __float_as_uint(v);
@@ -80,141 +87,211 @@ struct Sto<MSBG_STORAGE_BF16> {
 
 int elem_bytes(int storage) { return storage == MSBG_STORAGE_F32 ? 4 : 2; }
 
-struct Cell {
-    i64 c, xl, xr, yd, yu, zb, zf;
+// One row vector of W consecutive cells as stored (W = 4 for FP32, 8 for the
+// 16-bit types: 16 bytes, one load per lane either way).  The 16-bit types
+// stay packed in registers and are widened where a value is used, which keeps
+// the register cost of a column at half its FP32 size.
+template <int ST>
+struct Row {
+    static constexpr int W = ST == MSBG_STORAGE_F32 ? 4 : 8;
+    uint32_t w[4];
+    __device__ __forceinline__ float at(int i) const {
+        if constexpr (ST == MSBG_STORAGE_F32)
+            return __uint_as_float(w[i]);
+        else
+            return (i & 1) ? Sto<ST>::hi(w[i >> 1]) : Sto<ST>::lo(w[i >> 1]);
+    }
+    __device__ __forceinline__ void load(const typename Sto<ST>::T *p, i64 i) {  // i: multiple of W
+        const uint4 q = *reinterpret_cast<const uint4 *>(p + i);
+        w[0] = q.x;
+        w[1] = q.y;
+        w[2] = q.z;
+        w[3] = q.w;
+    }
+    __device__ __forceinline__ void store(typename Sto<ST>::T *p, i64 i) const {
+        *reinterpret_cast<uint4 *>(p + i) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __device__ __forceinline__ typename Sto<ST>::T bits(int i) const {
+        if constexpr (ST == MSBG_STORAGE_F32)
+            return __uint_as_float(w[i]);
+        else
+            return (typename Sto<ST>::T)((i & 1) ? w[i >> 1] >> 16 : w[i >> 1] & 0xffffu);
+    }
+    // Cell i takes v (rounded to the storage type); the other cells keep their bits.
+    __device__ __forceinline__ void set(int i, float v) {
+        if constexpr (ST == MSBG_STORAGE_F32) {
+            w[i] = __float_as_uint(v);
+        } else {
+            const uint32_t h = Sto<ST>::cv(v);
+            w[i >> 1] = (i & 1) ? (w[i >> 1] & 0xffffu) | h << 16 : (w[i >> 1] & 0xffff0000u) | h;
+        }
+    }
 };
 
-__device__ __forceinline__ Cell cell_at(const PBlock &B, const Dom &d, int tx, int ty, int tz, int gx, int gy, int gz) {
-    Cell k;
-    const int vid = tx | ty << 4 | tz << 8;
-    k.c = ((i64)B.slot << 12) + vid;
-    k.xl = tx > 0 ? k.c - 1 : (gx == 0 ? k.c : ((i64)B.nb[0] << 12) + (vid | 15));
-    k.xr = gx + 1 >= d.sx ? k.c : (tx < 15 ? k.c + 1 : ((i64)B.nb[1] << 12) + (vid & ~15));
-    k.yd = ty > 0 ? k.c - 16 : (gy == 0 ? k.c : ((i64)B.nb[2] << 12) + (vid | 0xf0));
-    k.yu = gy + 1 >= d.sy ? k.c : (ty < 15 ? k.c + 16 : ((i64)B.nb[3] << 12) + (vid & ~0xf0));
-    k.zb = tz > 0 ? k.c - 256 : (gz == 0 ? k.c : ((i64)B.nb[4] << 12) + (vid | 0xf00));
-    k.zf = gz + 1 >= d.sz ? k.c : (tz < 15 ? k.c + 256 : ((i64)B.nb[5] << 12) + (vid & ~0xf00));
-    return k;
-}
-
-template <int ST>
-__device__ __forceinline__ float residual_of(const typename Sto<ST>::T *x, const typename Sto<ST>::T *b, const Cell &k,
-                                             float &xc) {
-    using S = Sto<ST>;
-    xc = S::ld(x, k.c);
-    const float s = ((((S::ld(x, k.xl) + S::ld(x, k.xr)) + S::ld(x, k.yd)) + S::ld(x, k.yu)) + S::ld(x, k.zb)) +
-                    S::ld(x, k.zf);
-    return __builtin_fmaf(-6.0f, xc, S::ld(b, k.c) + s);
-}
-
-// A quad: 4 x-adjacent cells (x = 4q .. 4q+3) of one row of a listed block
-// with their 6 neighbours, resolved exactly as cell_at: a neighbour outside
-// the domain is the cell itself, across a block face it is read from the
-// face-neighbour slot.  A work-group (256 threads) covers 4 z-slices of one
-// block: thread = quad q (0..3) x row y (16) x slice (4).
-struct Quad {
-    float c[4], l, r, ym[4], yp[4], zm[4], zp[4];
-    i64 c0;
-    int gx0, gy, gz, n;  // n = cells of the quad inside the domain (0 = none)
+// A column: W x-adjacent cells (x = W q .. W q + W - 1) of one row, at Z
+// consecutive z, of a listed block, with their 6 neighbours resolved as in the
+// operator: a neighbour outside the domain is the cell itself, across a block
+// face it is read from the face-neighbour slot.  A work-group (256 threads)
+// covers G = 256 / (16 / W * 16) z-groups of Z slices; KS work-groups cover a
+// block.  Every value a thread needs is loaded up front (one 16-byte load per
+// row vector), which keeps enough bytes in flight per wave for HBM-sized bands.
+template <int ST, int Z>
+struct ColMap {
+    static constexpr int W = Row<ST>::W;
+    static constexpr int NQ = 16 / W;               // columns per row
+    static constexpr int G = kThreads / (NQ * 16);  // z-groups per work-group
+    static constexpr int KS = 16 / (G * Z);         // work-groups per block
+    static_assert(KS >= 1 && KS * G * Z == 16, "column shape must tile a 16^3 block");
 };
 
-template <int ST>
-__device__ __forceinline__ void load_quad(const PBlock &B, const Dom &d, const typename Sto<ST>::T *x, int wg,
-                                          Quad &Q) {
-    using S = Sto<ST>;
-    const int t = threadIdx.x, q = t & 3, ty = (t >> 2) & 15, tz = ((wg % kSlices) << 2) | (t >> 6);
-    Q.gx0 = B.ox + 4 * q;
+template <int ST, int Z>
+struct Col {
+    Row<ST> c[Z + 2];  // planes z0 - 1 .. z0 + Z; the cells are c[1 .. Z]
+    Row<ST> ym[Z], yp[Z], b[Z];  // b: right-hand side rows of the cells
+    float l[Z], r[Z];
+    i64 c0;                   // element index of cell (x0, y, z0)
+    int gx0, gy, gz0, n, nz;  // n / nz: cells inside the domain along x / z (0 = none)
+};
+
+// Column origin of this thread (element index c0, global cell gx0, gy, gz0,
+// in-block vid); false when the column lies outside the domain.
+template <int ST, int Z>
+__device__ __forceinline__ bool col_origin(const PBlock &B, const Dom &d, int wg, Col<ST, Z> &Q, int &vid) {
+    using M = ColMap<ST, Z>;
+    const int t = threadIdx.x, q = t % M::NQ, ty = (t / M::NQ) & 15;
+    const int tz0 = ((wg % M::KS) * M::G + t / (M::NQ * 16)) * Z;
+    Q.gx0 = B.ox + M::W * q;
     Q.gy = B.oy + ty;
-    Q.gz = B.oz + tz;
-    Q.n = 0;
-    if (Q.gx0 >= d.sx || Q.gy >= d.sy || Q.gz >= d.sz) return;
-    Q.n = min(4, d.sx - Q.gx0);
-    const int vid = 4 * q | ty << 4 | tz << 8;
-    const i64 base = (i64)B.slot << 12;
-    Q.c0 = base + vid;
-    S::ld4(x, Q.c0, Q.c);
-    Q.l = q > 0 ? S::ld(x, Q.c0 - 1) : (Q.gx0 == 0 ? Q.c[0] : S::ld(x, ((i64)B.nb[0] << 12) + (vid | 15)));
-    Q.r = Q.gx0 + 4 >= d.sx ? 0.0f : (q < 3 ? S::ld(x, Q.c0 + 4) : S::ld(x, ((i64)B.nb[1] << 12) + (vid & ~15)));
-    if (ty > 0)
-        S::ld4(x, Q.c0 - 16, Q.ym);
-    else if (Q.gy == 0)
-        for (int i = 0; i < 4; ++i) Q.ym[i] = Q.c[i];
-    else
-        S::ld4(x, ((i64)B.nb[2] << 12) + (vid | 0xf0), Q.ym);
-    if (Q.gy + 1 >= d.sy)
-        for (int i = 0; i < 4; ++i) Q.yp[i] = Q.c[i];
-    else if (ty < 15)
-        S::ld4(x, Q.c0 + 16, Q.yp);
-    else
-        S::ld4(x, ((i64)B.nb[3] << 12) + (vid & ~0xf0), Q.yp);
-    if (tz > 0)
-        S::ld4(x, Q.c0 - 256, Q.zm);
-    else if (Q.gz == 0)
-        for (int i = 0; i < 4; ++i) Q.zm[i] = Q.c[i];
-    else
-        S::ld4(x, ((i64)B.nb[4] << 12) + (vid | 0xf00), Q.zm);
-    if (Q.gz + 1 >= d.sz)
-        for (int i = 0; i < 4; ++i) Q.zp[i] = Q.c[i];
-    else if (tz < 15)
-        S::ld4(x, Q.c0 + 256, Q.zp);
-    else
-        S::ld4(x, ((i64)B.nb[5] << 12) + (vid & ~0xf00), Q.zp);
+    Q.gz0 = B.oz + tz0;
+    Q.n = Q.nz = 0;
+    if (Q.gx0 >= d.sx || Q.gy >= d.sy || Q.gz0 >= d.sz) return false;
+    Q.n = min(M::W, d.sx - Q.gx0);
+    Q.nz = min(Z, d.sz - Q.gz0);
+    vid = M::W * q | ty << 4 | tz0 << 8;
+    Q.c0 = ((i64)B.slot << 12) + vid;
+    return true;
 }
 
-// t = fmaf(-6, x_c, b_c + s) of cell i of the quad, s summed -x, +x, -y, +y, -z, +z.
-__device__ __forceinline__ float quad_residual(const Quad &Q, const Dom &d, int i, float bi) {
-    const float xl = i == 0 ? Q.l : Q.c[i - 1];
-    const float xr = Q.gx0 + i + 1 >= d.sx ? Q.c[i] : (i == 3 ? Q.r : Q.c[i + 1]);
-    const float s = ((((xl + xr) + Q.ym[i]) + Q.yp[i]) + Q.zm[i]) + Q.zp[i];
-    return __builtin_fmaf(-6.0f, Q.c[i], bi + s);
+// Every neighbour is a load from a selected address (a mirrored neighbour is
+// the cell itself, so it is loaded from the cell's own address), so no load
+// waits on another and the compiler issues them back to back.  Rows past the
+// domain (j >= nz) repeat the last in-domain row; their values are unused.
+template <int ST, int Z>
+__device__ __forceinline__ void load_col(const PBlock &B, const Dom &d, const typename Sto<ST>::T *x,
+                                         const typename Sto<ST>::T *b, int wg, Col<ST, Z> &Q) {
+    using M = ColMap<ST, Z>;
+    constexpr int W = M::W;
+    int vid;
+    if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
+    const int q = (vid & 15) / W, ty = (vid >> 4) & 15, tz0 = vid >> 8;
+    const i64 nb[6] = {(i64)B.nb[0] << 12, (i64)B.nb[1] << 12, (i64)B.nb[2] << 12,
+                       (i64)B.nb[3] << 12, (i64)B.nb[4] << 12, (i64)B.nb[5] << 12};
+    // z - 1 plane, z + Z plane (the latter read only when the column's last
+    // cell has an in-domain +z neighbour)
+    Q.c[0].load(x, tz0 > 0 ? Q.c0 - 256 : (Q.gz0 == 0 ? Q.c0 : nb[4] + (vid | 0xf00)));
+    Q.c[Z + 1].load(x, Q.gz0 + Z >= d.sz ? Q.c0 : (tz0 + Z <= 15 ? Q.c0 + 256 * Z : nb[5] + (vid & ~0xf00)));
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        const int jj = min(j, Q.nz - 1);
+        const i64 cj = Q.c0 + 256 * jj;
+        const int vj = vid + 256 * jj;
+        Q.c[j + 1].load(x, cj);
+        Q.b[j].load(b, cj);
+        Q.ym[j].load(x, ty > 0 ? cj - 16 : (Q.gy == 0 ? cj : nb[2] + (vj | 0xf0)));
+        Q.yp[j].load(x, Q.gy + 1 >= d.sy ? cj : (ty < 15 ? cj + 16 : nb[3] + (vj & ~0xf0)));
+        Q.l[j] = Sto<ST>::ld(x, q > 0 ? cj - 1 : (Q.gx0 == 0 ? cj : nb[0] + (vj | 15)));
+        Q.r[j] = Sto<ST>::ld(x, Q.gx0 + W >= d.sx ? cj : (q < M::NQ - 1 ? cj + W : nb[1] + (vj & ~15)));
+    }
+}
+
+// t = fmaf(-6, x_c, b_c + s) of cell (i, j) of the column, s summed -x, +x, -y, +y, -z, +z.
+template <int ST, int Z>
+__device__ __forceinline__ float col_residual(const Col<ST, Z> &Q, const Dom &d, int i, int j, float bi) {
+    constexpr int W = ColMap<ST, Z>::W;
+    const float xc = Q.c[j + 1].at(i);
+    const float xl = i == 0 ? Q.l[j] : Q.c[j + 1].at(i - 1);
+    const float xr = Q.gx0 + i + 1 >= d.sx ? xc : (i == W - 1 ? Q.r[j] : Q.c[j + 1].at(i + 1));
+    const float zp = Q.gz0 + j + 1 >= d.sz ? xc : Q.c[j + 2].at(i);
+    const float s = ((((xl + xr) + Q.ym[j].at(i)) + Q.yp[j].at(i)) + Q.c[j].at(i)) + zp;
+    return __builtin_fmaf(-6.0f, xc, bi + s);
 }
 
 // RB: in place, cells with (gx + gy + gz) & 1 == color (their neighbours all
-// have the other colour, so a launch never reads a cell it writes).
-// !RB: weighted Jacobi, x_in -> x_out (distinct buffers).
-template <int ST, bool RB>
+// have the other colour, so a launch never reads a cell it writes; the row
+// vector is written back whole, the other colour's cells with their loaded
+// bits).  !RB: weighted Jacobi, x_in -> x_out (distinct buffers; cells
+// outside the domain are not written).
+template <int ST, bool RB, int Z>
 __global__ __launch_bounds__(kThreads) void relax_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                          const typename Sto<ST>::T *xin, typename Sto<ST>::T *xout,
                                                          const typename Sto<ST>::T *b, float w6, int color) {
+    constexpr int W = ColMap<ST, Z>::W;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const PBlock B = blocks[wg / kSlices];
-    Quad Q;
-    load_quad<ST>(B, d, xin, wg, Q);
+    const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
+    Col<ST, Z> Q;
+    load_col<ST, Z>(B, d, xin, b, wg, Q);
     if (!Q.n) return;
-    float bq[4];
-    Sto<ST>::ld4(b, Q.c0, bq);
-    const int par = (Q.gx0 + Q.gy + Q.gz) & 1;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (i >= Q.n) break;
-        if (RB && ((par + i) & 1) != color) continue;
-        const float t = quad_residual(Q, d, i, bq[i]);
-        xout[Q.c0 + i] = Sto<ST>::cv(__builtin_fmaf(w6, t, Q.c[i]));
+    for (int j = 0; j < Z; ++j) {
+        if (j >= Q.nz) break;
+        const int par = (Q.gx0 + Q.gy + Q.gz0 + j) & 1;
+        Row<ST> v = Q.c[j + 1];
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+            if (i < Q.n && (!RB || ((par + i) & 1) == color))
+                v.set(i, __builtin_fmaf(w6, col_residual<ST, Z>(Q, d, i, j, Q.b[j].at(i)), Q.c[j + 1].at(i)));
+        const i64 cj = Q.c0 + 256 * j;
+        if (RB || Q.n == W) {
+            v.store(xout, cj);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+                if (i < Q.n) xout[cj + i] = v.bits(i);
+        }
     }
 }
 
 // Residual r = b - A x (r optional, FP32) and per-work-group FP64 partial sums
-// of r^2 and b^2 (fixed-order reduction: lane butterfly, then waves 0..3).
-template <int ST>
+// of r^2 and b^2 (fixed-order reduction: per thread cells in (z, x) order,
+// lane butterfly, then waves 0..3).
+template <int ST, int Z>
 __global__ __launch_bounds__(kThreads) void residual_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                             const typename Sto<ST>::T *x,
                                                             const typename Sto<ST>::T *b, float *r,
                                                             double2 *partial) {
+    constexpr int W = ColMap<ST, Z>::W;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
-    const PBlock B = blocks[wg / kSlices];
-    Quad Q;
-    load_quad<ST>(B, d, x, wg, Q);
+    const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
+    Col<ST, Z> Q;
+    load_col<ST, Z>(B, d, x, b, wg, Q);
     double rr = 0.0, bb = 0.0;
     if (Q.n) {
-        float bq[4];
-        Sto<ST>::ld4(b, Q.c0, bq);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (i >= Q.n) break;
-            const float t = quad_residual(Q, d, i, bq[i]);
-            if (r) r[Q.c0 + i] = t;
-            rr += (double)t * (double)t;
-            bb += (double)bq[i] * (double)bq[i];
+        for (int j = 0; j < Z; ++j) {
+            if (j >= Q.nz) break;
+            float t[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                t[i] = 0.0f;
+                if (i < Q.n) {
+                    const float bi = Q.b[j].at(i);
+                    t[i] = col_residual<ST, Z>(Q, d, i, j, bi);
+                    rr += (double)t[i] * (double)t[i];
+                    bb += (double)bi * (double)bi;
+                }
+            }
+            if (r) {
+                const i64 cj = Q.c0 + 256 * j;
+                if (Q.n == W) {
+#pragma unroll
+                    for (int k = 0; k < W; k += 4)
+                        *reinterpret_cast<float4 *>(r + cj + k) = make_float4(t[k], t[k + 1], t[k + 2], t[k + 3]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        if (i < Q.n) r[cj + i] = t[i];
+                }
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -273,40 +350,77 @@ __global__ __launch_bounds__(256) void reduce_norms_kernel(const double2 *__rest
     }
 }
 
-// Refinement step start: rt = T(r / scale), e = 0 on the listed cells.
-template <int ST>
+// Refinement step start: rt = T(r / scale), e = 0 on the listed cells
+// (column mapping of the relaxation, row vectors where the row is whole).
+template <int ST, int Z>
 __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__restrict__ blocks, Dom d, const float *r,
                                                             const float *scale, typename Sto<ST>::T *rt,
                                                             typename Sto<ST>::T *e) {
-    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int gx = B.ox + tx, gy = B.oy + ty;
-    if (gx >= d.sx || gy >= d.sy) return;
+    constexpr int W = ColMap<ST, Z>::W;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
+    Col<ST, Z> Q;
+    int vid;
+    if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
     const float inv = scale[1];
-    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
-    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
-        if (B.oz + tz >= d.sz) break;
-        const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);
-        rt[c] = Sto<ST>::cv(r[c] * inv);
-        e[c] = Sto<ST>::cv(0.0f);
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        if (j >= Q.nz) break;
+        const i64 cj = Q.c0 + 256 * j;
+        if (Q.n == W) {
+            Row<ST> v, z;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v.w[k] = z.w[k] = 0u;  // +0 in every storage type
+#pragma unroll
+            for (int k = 0; k < W; k += 4) {
+                const float4 q = *reinterpret_cast<const float4 *>(r + cj + k);
+                v.set(k, q.x * inv);
+                v.set(k + 1, q.y * inv);
+                v.set(k + 2, q.z * inv);
+                v.set(k + 3, q.w * inv);
+            }
+            v.store(rt, cj);
+            z.store(e, cj);
+        } else {
+            for (int i = 0; i < Q.n; ++i) {
+                rt[cj + i] = Sto<ST>::cv(r[cj + i] * inv);
+                e[cj + i] = Sto<ST>::cv(0.0f);
+            }
+        }
     }
 }
 
 // Refinement step end: x = fmaf(scale, e, x) on the listed cells.
-template <int ST>
+template <int ST, int Z>
 __global__ __launch_bounds__(kThreads) void ir_update_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                              const typename Sto<ST>::T *e, const float *scale,
                                                              float *x) {
-    const PBlock B = blocks[xcd_remap(blockIdx.x, gridDim.x) / kSlices];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int gx = B.ox + tx, gy = B.oy + ty;
-    if (gx >= d.sx || gy >= d.sy) return;
+    constexpr int W = ColMap<ST, Z>::W;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
+    Col<ST, Z> Q;
+    int vid;
+    if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
     const float s = scale[0];
-    const int z0 = (xcd_remap(blockIdx.x, gridDim.x) % kSlices) * (16 / kSlices);
-    for (int tz = z0; tz < z0 + 16 / kSlices; ++tz) {
-        if (B.oz + tz >= d.sz) break;
-        const i64 c = ((i64)B.slot << 12) + (tx | ty << 4 | tz << 8);
-        x[c] = __builtin_fmaf(s, Sto<ST>::ld(e, c), x[c]);
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        if (j >= Q.nz) break;
+        const i64 cj = Q.c0 + 256 * j;
+        if (Q.n == W) {
+            Row<ST> ev;
+            ev.load(e, cj);
+#pragma unroll
+            for (int k = 0; k < W; k += 4) {
+                float4 q = *reinterpret_cast<const float4 *>(x + cj + k);
+                q.x = __builtin_fmaf(s, ev.at(k), q.x);
+                q.y = __builtin_fmaf(s, ev.at(k + 1), q.y);
+                q.z = __builtin_fmaf(s, ev.at(k + 2), q.z);
+                q.w = __builtin_fmaf(s, ev.at(k + 3), q.w);
+                *reinterpret_cast<float4 *>(x + cj + k) = q;
+            }
+        } else {
+            for (int i = 0; i < Q.n; ++i) x[cj + i] = __builtin_fmaf(s, Sto<ST>::ld(e, cj + i), x[cj + i]);
+        }
     }
 }
 
@@ -321,6 +435,31 @@ __global__ void convert_kernel(const typename Sto<SS>::T *src, typename Sto<DS>:
         dst[i] = Sto<DS>::cv(Sto<SS>::ld(src, i));
 }
 
+// Column depth Z of the relaxation / residual / refinement kernels of storage
+// `st` (p->zcol), as a compile-time constant for f.
+template <int ST, class F>
+void with_z(const msbg_poisson *p, F &&f) {
+    const int z = p->zcol[ST];
+    if constexpr (ST == MSBG_STORAGE_F32) {
+        if (z == 4)
+            f(std::integral_constant<int, 4>{});
+        else if (z == 2)
+            f(std::integral_constant<int, 2>{});
+        else
+            f(std::integral_constant<int, 1>{});
+    } else {
+        if (z == 2)
+            f(std::integral_constant<int, 2>{});
+        else
+            f(std::integral_constant<int, 1>{});
+    }
+}
+
+template <int ST, int Z>
+int col_groups(const msbg_poisson *p) {
+    return (int)(p->n_active * ColMap<ST, Z>::KS);
+}
+
 }  // namespace
 }  // namespace msbg
 
@@ -329,17 +468,25 @@ using namespace msbg;
 namespace msbg {
 namespace pband {
 
+template <int ST>
+void residual_st(msbg_poisson *p, hipStream_t s, const void *x, const void *b, float *r) {
+    using T = typename Sto<ST>::T;
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        const int n = col_groups<ST, Z>(p);
+        residual_kernel<ST, Z><<<n, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)x, (const T *)b, r,
+                                                      p->d_partial);
+        reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, n, (double)p->n_cells, p->d_norm, p->d_scale);
+    });
+}
+
 void residual(msbg_poisson *p, hipStream_t s, int st, const void *x, const void *b, float *r) {
     if (st == MSBG_STORAGE_F32)
-        residual_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)x,
-                                                                        (const float *)b, r, p->d_partial);
+        residual_st<MSBG_STORAGE_F32>(p, s, x, b, r);
     else if (st == MSBG_STORAGE_F16)
-        residual_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(
-            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
+        residual_st<MSBG_STORAGE_F16>(p, s, x, b, r);
     else
-        residual_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(
-            p->d_blocks, p->dom, (const uint16_t *)x, (const uint16_t *)b, r, p->d_partial);
-    reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, p->n_wg, (double)p->n_cells, p->d_norm, p->d_scale);
+        residual_st<MSBG_STORAGE_BF16>(p, s, x, b, r);
 }
 
 int read_norms(msbg_poisson *p, double *r2, double *b2) {
@@ -374,12 +521,16 @@ void launch_relax(msbg_poisson *p, hipStream_t s, const void *xin, void *xout, c
                   bool rb) {
     using T = typename Sto<ST>::T;
     const float w6 = omega / 6.0f;
-    if (rb)
-        relax_kernel<ST, true><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        const int n = col_groups<ST, Z>(p);
+        if (rb)
+            relax_kernel<ST, true, Z><<<n, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
+                                                             (const T *)b, w6, color);
+        else
+            relax_kernel<ST, false, Z><<<n, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
                                                               (const T *)b, w6, color);
-    else
-        relax_kernel<ST, false><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)xin, (T *)xout,
-                                                               (const T *)b, w6, color);
+    });
 }
 
 void relax(msbg_poisson *p, hipStream_t s, int st, const void *xin, void *xout, const void *b, float omega, int color,
@@ -399,28 +550,42 @@ void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, f
     }
 }
 
+template <int ST>
+void ir_scale_st(msbg_poisson *p, hipStream_t s) {
+    using T = typename Sto<ST>::T;
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        ir_scale_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
+                                                                          (T *)p->d_rt, (T *)p->d_e);
+    });
+}
+
+template <int ST>
+void ir_update_st(msbg_poisson *p, hipStream_t s, float *x) {
+    using T = typename Sto<ST>::T;
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        ir_update_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)p->d_e,
+                                                                           p->d_scale, x);
+    });
+}
+
 void ir_scale(msbg_poisson *p, hipStream_t s, int st) {
     if (st == MSBG_STORAGE_F32)
-        ir_scale_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
-                                                                        (float *)p->d_rt, (float *)p->d_e);
+        ir_scale_st<MSBG_STORAGE_F32>(p, s);
     else if (st == MSBG_STORAGE_F16)
-        ir_scale_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
-                                                                        (uint16_t *)p->d_rt, (uint16_t *)p->d_e);
+        ir_scale_st<MSBG_STORAGE_F16>(p, s);
     else
-        ir_scale_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
-                                                                         (uint16_t *)p->d_rt, (uint16_t *)p->d_e);
+        ir_scale_st<MSBG_STORAGE_BF16>(p, s);
 }
 
 void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x) {
     if (st == MSBG_STORAGE_F32)
-        ir_update_kernel<MSBG_STORAGE_F32><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom, (const float *)p->d_e,
-                                                                         p->d_scale, x);
+        ir_update_st<MSBG_STORAGE_F32>(p, s, x);
     else if (st == MSBG_STORAGE_F16)
-        ir_update_kernel<MSBG_STORAGE_F16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom,
-                                                                         (const uint16_t *)p->d_e, p->d_scale, x);
+        ir_update_st<MSBG_STORAGE_F16>(p, s, x);
     else
-        ir_update_kernel<MSBG_STORAGE_BF16><<<p->n_wg, kThreads, 0, s>>>(p->d_blocks, p->dom,
-                                                                          (const uint16_t *)p->d_e, p->d_scale, x);
+        ir_update_st<MSBG_STORAGE_BF16>(p, s, x);
 }
 
 // Inner fields for `storage`; slots outside the listed blocks stay zero (the
@@ -537,6 +702,10 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
     }
     p->n_cells = cells;
     p->n_wg = (int)(n_active * kSlices);
+    // Column depth: the deepest column (fewest work-groups, most bytes in
+    // flight per wave) that still leaves >= 2048 work-groups per launch.
+    p->zcol[MSBG_STORAGE_F32] = n_active >= 2048 ? 4 : (n_active >= 1024 ? 2 : 1);
+    p->zcol[MSBG_STORAGE_F16] = p->zcol[MSBG_STORAGE_BF16] = n_active >= 2048 ? 2 : 1;
     p->nx = layout->nx;
     p->ny = layout->ny;
     p->nz = layout->nz;
