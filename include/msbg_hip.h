@@ -169,6 +169,19 @@ int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t 
 /* regularize_refinement_map -- multires/refinement.rs:66-107 on a device
  * level map of nx*ny*nz bytes, in place. */
 int msbg_regularize_levels(uint8_t *dev_levels, int64_t nx, int64_t ny, int64_t nz, int n_levels);
+/* compute_block_topology -- multires/refinement.rs:124-208 (levelMg 0, no
+ * obstacles) on a device level map of nx*ny*nz bytes: d_flags_out[b] (device,
+ * u16) = d_user_flags[b] (device, or NULL for none) | BLK_DOM_BORDER (1 << 1)
+ * on the block-grid faces | BLK_FINE_COARSE (1 << 3) if a 26-neighbour is
+ * coarser | BLK_COARSE_FINE (1 << 2) if one is finer (blockinfo.rs:39-54).
+ * d_fc_bids_out (device, room for nx*ny*nz) receives Topology::
+ * blocks_fine_coarse, level 0 first, ascending within a level;
+ * fc_counts[0..n_levels) (host) their lengths; *n_act (host) =
+ * Topology::n_act_blocks, the BLK_EXISTS (1 << 12) count.  The reference's
+ * store.level0 is the level map itself. */
+int msbg_block_topology(const uint8_t *d_levels, int64_t nx, int64_t ny, int64_t nz, int n_levels,
+                        const uint16_t *d_user_flags, uint16_t *d_flags_out, int64_t *d_fc_bids_out,
+                        int64_t *fc_counts, int64_t *n_act);
 
 /* ---- particles ---------------------------------------------------------------
  * Particle arrays are device pointers: positions and velocities are n x 3
@@ -260,6 +273,32 @@ int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *ac
  * of active blocks. */
 int msbg_density_splat(msbg_grid *g, uint16_t *d_slab16, int64_t slab_slots, const float *d_pos, int64_t n,
                        float r_scan, int64_t *n_active);
+
+/* ---- surface-reconstruction tail (particles/mod.rs:214-302) ------------------
+ * place -- particles/sort.rs:31-143: instance i (0 <= i < n_instances; 0 = one
+ * per base point, n_instances <= n_base) of the base cloud d_base (device,
+ * n_base x 3) sits at 0.2 smax + 0.6 smax (b_i - bbox_min) / span_max and
+ * holds every base point j at origin + smin instance_scale_factor (b_j -
+ * bbox_min) / span_max (smin / smax: the smallest / largest domain edge, f32
+ * operations in the reference's order, none fused).  A point is kept when it
+ * lies within DomainBounds (align24(0.005) .. align24(size - 0.005),
+ * particles/mod.rs:180-199) and its truncation lies in the grid.  Kept
+ * positions (device, n x 3) and centre block ids (device) are written in
+ * instance-major, base-point order; *n_out = their count (MSBG_ENOMEM when it
+ * exceeds `capacity`, *n_out still set; with both outputs NULL and capacity 0
+ * only the count is returned).  Placed::active is the footprint
+ * union of the kept positions: msbg_active_blocks with r_scan. */
+int msbg_surface_place(const float *d_base, int64_t n_base, const float *bbox_min, float span_max, int64_t sx,
+                       int64_t sy, int64_t sz, int64_t n_instances, float instance_scale_factor, float *d_pos_out,
+                       int32_t *d_bids_out, int64_t capacity, int64_t *n_out);
+/* finalize -- particles/finalize.rs:35-71 on the Density (u16) slab of a
+ * layout grid, for the `active` (host) blocks that have a value slot:
+ * ratio = u / 65535 (times the f32 constant 1/65535); touched = ratio < 0.999;
+ * f = 1 - clamp((sqrt(ratio r_scan^2) - r_p + r_p) / (2 r_p + nb_dist), 0, 1)
+ * with r_scan = r_p + nb_dist; stored as round(f * 65535) (half away from
+ * zero), 0 where untouched. */
+int msbg_density_finalize(const msbg_grid *layout, uint16_t *d_slab16, int64_t slab_slots, const int64_t *active,
+                          int64_t n, float r_particle, float nb_dist);
 
 /* ---- Poisson relaxation on a band of value blocks ------------------------
  * No reference counterpart: the reference's solver is planned, not built

@@ -12,6 +12,8 @@ from .lib import MsbgError, DeviceArray  # noqa: F401
 from .device import (  # noqa: F401
     BoundaryCondition, DeviceGrid, PdeParams, Stencil, SweepPlan, Sweeper, apply_pde, color_buckets,
     downsample_channel_avg, halo_fill, halo_fill_multires, halo_topology, regularize_refinement_map, sweep_multires,
+    compute_block_topology, Topology,
 )
 from .solve import PoissonBand, Storage  # noqa: F401
 from .learned import CorrectorNet  # noqa: F401
+from .surface import SurfaceConfig, reconstruct_surface, reconstruct_and_smooth  # noqa: F401

@@ -9,8 +9,8 @@ namespace msbg {
 // ---------------------------------------------------------------------------
 // Workspace.
 // ---------------------------------------------------------------------------
-static void *g_ws[16] = {nullptr};
-static size_t g_ws_size[16] = {0};
+static void *g_ws[24] = {nullptr};
+static size_t g_ws_size[24] = {0};
 
 void *workspace(int id, size_t bytes, int *rc) {
     *rc = MSBG_OK;

@@ -255,3 +255,33 @@ def downsample_channel_avg(coarse: DeviceGrid, fine: DeviceGrid, bids):
 def regularize_refinement_map(levels: DeviceArray, nx, ny, nz, n_levels):
     """multires/refinement.rs:66-107, in place on a device level map."""
     levels.lib.call("msbg_regularize_levels", levels.ptr, nx, ny, nz, n_levels)
+
+
+class Topology:
+    """multires/refinement.rs:110-116: FINE_COARSE blocks per level (host
+    int64 arrays, ascending) and the EXISTS block count."""
+
+    def __init__(self, blocks_fine_coarse, n_act_blocks):
+        self.blocks_fine_coarse = blocks_fine_coarse
+        self.n_act_blocks = n_act_blocks
+
+
+def compute_block_topology(levels: DeviceArray, nx, ny, nz, n_levels, block_flags: DeviceArray = None):
+    """compute_block_topology (multires/refinement.rs:124-208) on a device
+    level map: (device u16 block flags -- the reference's store.flags[0] --,
+    Topology).  store.level0 is the level map itself."""
+    lib = levels.lib
+    nb = nx * ny * nz
+    flags = lib.empty(nb, np.uint16)
+    fc = lib.empty(nb, np.int64)
+    counts = np.zeros(n_levels, np.int64)
+    n_act = C.c_int64()
+    lib.call("msbg_block_topology", levels.ptr, nx, ny, nz, n_levels,
+             None if block_flags is None else block_flags.ptr, flags.ptr, fc.ptr, counts.ctypes.data,
+             C.byref(n_act))
+    allfc = fc.get()
+    lists, k = [], 0
+    for c in counts:
+        lists.append(allfc[k:k + int(c)].copy())
+        k += int(c)
+    return flags, Topology(lists, n_act.value)

@@ -75,6 +75,9 @@ _SIGS = {
     "msbg_p2g_bspline": [i32, vp, vp, vp, vp, vp, i64, C.POINTER(C.c_int64)],
     "msbg_g2p_bspline": [i32, vp, vp, i64, vp],
     "msbg_density_splat": [vp, vp, i64, vp, i64, f32, C.POINTER(C.c_int64)],
+    "msbg_surface_place": [vp, i64, vp, f32, i64, i64, i64, i64, f32, vp, vp, i64, C.POINTER(C.c_int64)],
+    "msbg_density_finalize": [vp, vp, i64, vp, i64, f32, f32],
+    "msbg_block_topology": [vp, i64, i64, i64, i32, vp, vp, vp, vp, C.POINTER(C.c_int64)],
     "msbg_mac_divergence": [vp, vp, vp, i64],
     "msbg_poisson_new": [vp, vp, i64, PP],
     "msbg_poisson_free": [vp],

@@ -83,6 +83,11 @@ def load():
     _sig(lib, "orc_net_loss_grad", C.c_double, f32p, f32p, f32p, i64, i32, f64p)
     _sig(lib, "orc_net_input", None, i64, i64, i64, i32p, i64p, i64, f32p, f32, f32p, i32p,
          np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"))
+    u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+    _sig(lib, "orc_domain_bounds", None, i64, i64, i64, f32p)
+    _sig(lib, "orc_surface_place", i64, f32p, i64, f32p, f32, i64, i64, i64, i64, f32, f32p, i64p, i64)
+    _sig(lib, "orc_density_finalize", None, i32p, u16p, i64p, i64, f32, f32)
+    _sig(lib, "orc_block_topology", i64, i64, i64, i64, u8p, i32, vp, u16p, i64p, i64p)
     _LIB = Oracle(lib)
     return _LIB
 
