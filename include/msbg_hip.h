@@ -90,6 +90,15 @@ int msbg_grid_read_blocks(const msbg_grid *g, const int64_t *bids, int64_t n, fl
 /* Bulk load / store of the whole layout (host b2s[n_blocks], slab[n_slots*B^3]). */
 int msbg_grid_load(msbg_grid *g, const int32_t *b2s, const float *slab, int64_t n_slots);
 int msbg_grid_store(const msbg_grid *g, int32_t *b2s, float *slab, int64_t cap_slots, int64_t *n_slots);
+/* The bid -> slot table alone (host b2s[n_blocks]; no device traffic):
+ * slot >= 2 is a value block (is_value_block, sparse_grid.rs:199), 0 / 1 the
+ * empty / full dummies. */
+int msbg_grid_block_slots(const msbg_grid *g, int32_t *b2s);
+/* Deep copy of src's layout and values into dst (same dimensions and block
+ * size, dst capacity >= src's slots), stream-ordered on the device.  No
+ * reference counterpart (SparseGrid has no Clone); it keeps the
+ * pre-projection face velocities of a FLIP step. */
+int msbg_grid_copy(msbg_grid *dst, const msbg_grid *src);
 /* value_block_ptr_mut -- sparse_grid.rs:424-429: device payload, NULL for dummies. */
 int msbg_grid_value_block_ptr(msbg_grid *g, int64_t bid, float **dev_ptr);
 /* Device views of the slab and the bid->slot table. */
@@ -257,6 +266,18 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
  * domain.  Each cell is stored at its layout index (the reference's running
  * index, mac.rs:155-166, coincides with it for whole 16^3 blocks). */
 int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *active, int64_t n);
+/* Pressure projection of the face velocities.  No reference counterpart: the
+ * reference stops at the divergence (mac.rs:134-171; its projection is the
+ * planned step 18).  For every cell c of the listed blocks (host bids) and
+ * axis a, the face u_a(c) between c and c + e_a (the layout
+ * msbg_mac_divergence reads) becomes u_a(c) - scale (p(c + e_a) - p(c)),
+ * the product and difference rounded separately.  p is a field over
+ * p_layout's slots (a PoissonBand field): a neighbour outside the domain is
+ * the cell itself (zero gradient), one in an absent block reads slot 0.  With
+ * A p = div (the band's operator) and scale = -1 the projected field's
+ * divergence equals div - A p, the solve's residual. */
+int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
+                               const int64_t *active, int64_t n, float scale);
 /* particles::splat -- particles/splat.rs:92-319 with D = Density (u16,
  * channel.rs:303-375): the active blocks of the particles (footprint r_scan,
  * active.rs:15-47) are ensured in the layout grid g and filled with 65535 in

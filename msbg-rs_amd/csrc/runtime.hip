@@ -358,6 +358,29 @@ int msbg_grid_store(const msbg_grid *g, int32_t *b2s, float *slab, int64_t cap_s
     return MSBG_OK;
 }
 
+int msbg_grid_copy(msbg_grid *dst, const msbg_grid *src) {
+    MSBG_CHECK(dst && src, MSBG_EINVAL, "msbg_grid_copy: null argument");
+    MSBG_CHECK(dst->sx == src->sx && dst->sy == src->sy && dst->sz == src->sz && dst->L == src->L, MSBG_EINVAL,
+               "msbg_grid_copy: grids must share dimensions and block size");
+    MSBG_CHECK(dst->capacity_slots >= src->n_slots, MSBG_ENOMEM, "msbg_grid_copy: destination capacity too small");
+    if (dst == src) return MSBG_OK;
+    MSBG_HIP(hipMemcpyAsync(dst->d_b2s, src->d_b2s, sizeof(int32_t) * src->n_blocks, hipMemcpyDeviceToDevice,
+                            g_stream));
+    MSBG_HIP(hipMemcpyAsync(dst->d_slab, src->d_slab, sizeof(float) * src->n_slots * src->N,
+                            hipMemcpyDeviceToDevice, g_stream));
+    dst->b2s_host = src->b2s_host;
+    dst->n_slots = src->n_slots;
+    dst->empty_value = src->empty_value;
+    dst->full_value = src->full_value;
+    return MSBG_OK;
+}
+
+int msbg_grid_block_slots(const msbg_grid *g, int32_t *b2s) {
+    MSBG_CHECK(g && b2s, MSBG_EINVAL, "msbg_grid_block_slots: null argument");
+    std::memcpy(b2s, g->b2s_host.data(), sizeof(int32_t) * g->n_blocks);
+    return MSBG_OK;
+}
+
 int msbg_grid_value_block_ptr(msbg_grid *g, int64_t bid, float **p) {
     int rc = check_bids(g, &bid, 1, "msbg_grid_value_block_ptr");
     if (rc) return rc;

@@ -702,8 +702,62 @@ __global__ __launch_bounds__(256) void divergence_kernel(GridView u, GridView v,
     d.slab[((int64_t)slot << 12) + vid] = ((ux1 - ux0) + (vy1 - vy0)) + (wz1 - wz0);
 }
 
+// Pressure projection of the face velocities: face a of cell c takes
+// u_a(c) - scale (p(c + e_a) - p(c)), the neighbour outside the domain being
+// the cell itself; one thread per cell, the three faces it owns.
+__global__ __launch_bounds__(256) void subtract_gradient_kernel(GridView u, GridView v, GridView w, GridView p,
+                                                                const int32_t *__restrict__ bids, int64_t n,
+                                                                float scale) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n * 4096) return;
+    const int64_t bid = bids[i >> 12];
+    const int vid = (int)(i & 4095);
+    const int x = (int)(bid % p.nx) * 16 + (vid & 15), y = (int)((bid / p.nx) % p.ny) * 16 + ((vid >> 4) & 15),
+              z = (int)(bid / ((int64_t)p.nx * p.ny)) * 16 + (vid >> 8);
+    if (x >= p.sx || y >= p.sy || z >= p.sz) return;
+    const float pc = voxel_at(p, x, y, z);
+    const float pn[3] = {x + 1 < p.sx ? voxel_at(p, x + 1, y, z) : pc, y + 1 < p.sy ? voxel_at(p, x, y + 1, z) : pc,
+                         z + 1 < p.sz ? voxel_at(p, x, y, z + 1) : pc};
+    const GridView ch[3] = {u, v, w};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int slot = ch[a].b2s[bid];
+        if (slot < 2) continue;
+        float *f = ch[a].slab + ((int64_t)slot << 12) + vid;
+        const float g = pn[a] - pc;
+        const float t = scale * g;
+        *f = *f - t;
+    }
+}
+
 }  // namespace
 }  // namespace msbg
+
+extern "C" int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
+                                          const int64_t *active, int64_t n, float scale) {
+    MSBG_CHECK(vel && vel[0] && vel[1] && vel[2] && p_layout && d_p, MSBG_EINVAL, "subtract gradient: null argument");
+    MSBG_CHECK(n >= 0 && (n == 0 || active), MSBG_EINVAL, "subtract gradient: bad active list");
+    for (int a = 0; a < 3; ++a)
+        MSBG_CHECK(vel[a]->L == 4 && vel[a]->sx == p_layout->sx && vel[a]->sy == p_layout->sy &&
+                       vel[a]->sz == p_layout->sz,
+                   MSBG_EINVAL, "subtract gradient: grids must share dimensions and 16^3 blocks");
+    MSBG_CHECK(p_layout->L == 4, MSBG_EINVAL, "subtract gradient: 16^3-block pressure layout required");
+    if (!n) return MSBG_OK;
+    for (int64_t k = 0; k < n; ++k)
+        MSBG_CHECK(active[k] >= 0 && active[k] < p_layout->n_blocks, MSBG_EINVAL, "subtract gradient: bid out of range");
+    int rc;
+    int32_t *d_bids = (int32_t *)workspace(19, sizeof(int32_t) * n, &rc);
+    if (rc) return rc;
+    std::vector<int32_t> h(active, active + n);
+    MSBG_HIP(hipMemcpyAsync(d_bids, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream()));
+    GridView pv = p_layout->view();
+    pv.slab = const_cast<float *>(d_p);
+    subtract_gradient_kernel<<<(int)((n * 4096 + 255) / 256), 256, 0, stream()>>>(vel[0]->view(), vel[1]->view(),
+                                                                                  vel[2]->view(), pv, d_bids, n, scale);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
 
 extern "C" int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *active, int64_t n) {
     MSBG_CHECK(vel && vel[0] && vel[1] && vel[2] && dst, MSBG_EINVAL, "divergence: null grid");

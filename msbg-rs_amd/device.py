@@ -81,6 +81,20 @@ class DeviceGrid:
         host._full_value = slab[1, 0]
         return host
 
+    def copy_from(self, src: "DeviceGrid"):
+        """Deep copy of src's layout and values (msbg_grid_copy)."""
+        self.lib.call("msbg_grid_copy", self.h, src.h)
+
+    def block_slots(self):
+        """Host copy of the bid -> slot table (no device traffic)."""
+        b2s = np.zeros(self.n_blocks, np.int32)
+        self.lib.call("msbg_grid_block_slots", self.h, b2s.ctypes.data)
+        return b2s
+
+    def value_blocks(self):
+        """Sorted bids of the value blocks (slot >= 2)."""
+        return np.nonzero(self.block_slots() >= 2)[0].astype(np.int64)
+
     def dims(self):
         d = np.zeros(8, np.int64)
         self.lib.call("msbg_grid_dims", self.h, d.ctypes.data)

@@ -43,6 +43,8 @@ _SIGS = {
     "msbg_grid_read_blocks": [vp, vp, i64, vp],
     "msbg_grid_load": [vp, vp, vp, i64],
     "msbg_grid_store": [vp, vp, vp, i64, vp],
+    "msbg_grid_block_slots": [vp, vp],
+    "msbg_grid_copy": [vp, vp],
     "msbg_grid_value_block_ptr": [vp, i64, PP],
     "msbg_grid_device_views": [vp, PP, PP],
     "msbg_color_buckets": [vp, vp, i64, vp, vp],
@@ -79,6 +81,7 @@ _SIGS = {
     "msbg_density_finalize": [vp, vp, i64, vp, i64, f32, f32],
     "msbg_block_topology": [vp, i64, i64, i64, i32, vp, vp, vp, vp, C.POINTER(C.c_int64)],
     "msbg_mac_divergence": [vp, vp, vp, i64],
+    "msbg_mac_subtract_gradient": [vp, vp, vp, vp, i64, f32],
     "msbg_poisson_new": [vp, vp, i64, PP],
     "msbg_poisson_free": [vp],
     "msbg_poisson_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],

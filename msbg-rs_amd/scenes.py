@@ -217,3 +217,29 @@ def poisson_rhs(grid: SparseGrid, active, seed=11):
     u = lcg_uniform(seed, active.size * grid.N).reshape(active.size, grid.N)
     b[grid.b2s[active]] = (u * np.float32(2.0) - np.float32(1.0)).astype(np.float32)
     return b
+
+
+# ---------------------------------------------------------------------------
+# Config 5: dam-break box + droplets (the FLIP step's particles).
+def dam_break(S, n, seed=21, box=(0.35, 0.6, 0.5), n_drops=6, drop_frac=0.1):
+    """n particles: (1 - drop_frac) n uniform in the box [0, box * S) at the
+    domain's low corner (at rest plus N(0, 0.1) jitter), the rest in n_drops
+    spheres of radius S/24 above it falling at -1 (plus the same jitter);
+    unit masses.  Positions lie in [0.5, S - 0.5) per axis."""
+    n_drop = int(n * drop_frac)
+    n_box = n - n_drop
+    u = lcg_uniform(seed, 3 * n_box).reshape(n_box, 3).astype(np.float64)
+    pos_box = 0.5 + u * (np.array(box) * S - 0.5)
+    c = lcg_uniform(seed + 1, 3 * n_drops).reshape(n_drops, 3).astype(np.float64)
+    centres = np.stack([S * (0.45 + 0.4 * c[:, 0]), S * (0.7 + 0.2 * c[:, 1]), S * (0.15 + 0.7 * c[:, 2])], 1)
+    r = S / 24.0
+    v = lcg_normal(seed + 2, 3 * n_drop).reshape(n_drop, 3).astype(np.float64)
+    v /= np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-12)
+    rad = r * lcg_uniform(seed + 3, n_drop).astype(np.float64) ** (1.0 / 3.0)
+    pick = np.arange(n_drop) % max(n_drops, 1)
+    pos_drop = centres[pick] + v * rad[:, None]
+    pos = np.clip(np.concatenate([pos_box, pos_drop]), 0.5, S - 0.5 - 1e-3).astype(np.float32)
+    vel = (0.1 * lcg_normal(seed + 4, 3 * n).reshape(n, 3)).astype(np.float32)
+    vel[n_box:, 1] -= np.float32(1.0)
+    mass = np.ones(n, np.float32)
+    return pos, vel, mass

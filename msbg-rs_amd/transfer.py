@@ -63,6 +63,13 @@ class MacGrid:
         a = np.ascontiguousarray(active, np.int64)
         self.lib.call("msbg_mac_divergence", _handles(self.velocity), dst.h, a.ctypes.data, a.size)
 
+    def subtract_gradient(self, p_layout: DeviceGrid, p, active, scale=1.0):
+        """Face velocities of the `active` blocks minus scale * grad p (p a
+        field over p_layout's slots, e.g. a PoissonBand solution)."""
+        a = np.ascontiguousarray(active, np.int64)
+        self.lib.call("msbg_mac_subtract_gradient", _handles(self.velocity), p_layout.h, p.ptr, a.ctypes.data,
+                      a.size, scale)
+
 
 def particles_to_grid(mac: MacGrid, pos, vel, mass, r_p) -> int:
     """fluid/p2g.rs:141-164; returns the number of active blocks."""

@@ -584,3 +584,45 @@ extern "C" void orc_mac_divergence(i64 sx, i64 sy, i64 sz, const float *u, const
                 }
     }
 }
+
+// ===========================================================================
+// Pressure projection of the face velocities (no reference counterpart: the
+// reference stops at the divergence, mac.rs:134-171, the projection being its
+// planned step 18).  Face a of cell c (the face between c and c + e_a, the
+// layout `divergence` reads) takes u_a(c) - scale * (p(c + e_a) - p(c)); a
+// neighbour outside the domain is the cell itself (zero gradient, the
+// Poisson operator's mirror), a neighbour in an absent block reads p's slot 0.
+// With A p = -div (or scale = -1 and A p = div) the new divergence is the
+// solve's residual.  Every cell of the listed blocks' clipped extent, in
+// place on each channel (the three channels' own bid -> slot tables).
+// ===========================================================================
+extern "C" void orc_mac_subtract_gradient(i64 sx, i64 sy, i64 sz, float *u, const i32 *ub2s, float *v,
+                                          const i32 *vb2s, float *w, const i32 *wb2s, const float *p,
+                                          const i32 *pb2s, const i64 *active, i64 n, float scale) {
+    const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nxy = nx * ny;
+    auto idx = [&](const i32 *b2s, i64 x, i64 y, i64 z) {
+        return (i64)b2s[(x >> 4) + (y >> 4) * nx + (z >> 4) * nxy] * 4096 + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8));
+    };
+    float *ch[3] = {u, v, w};
+    const i32 *cb[3] = {ub2s, vb2s, wb2s};
+    for (i64 k = 0; k < n; ++k) {
+        const i64 bid = active[k];
+        const i64 gx0 = (bid % nx) * 16, gy0 = ((bid / nx) % ny) * 16, gz0 = (bid / nxy) * 16;
+        const i64 gx1 = std::min(gx0 + 16, sx), gy1 = std::min(gy0 + 16, sy), gz1 = std::min(gz0 + 16, sz);
+        for (i64 z = gz0; z < gz1; ++z)
+            for (i64 y = gy0; y < gy1; ++y)
+                for (i64 x = gx0; x < gx1; ++x) {
+                    const float pc = p[idx(pb2s, x, y, z)];
+                    const float pn[3] = {x + 1 < sx ? p[idx(pb2s, x + 1, y, z)] : pc,
+                                         y + 1 < sy ? p[idx(pb2s, x, y + 1, z)] : pc,
+                                         z + 1 < sz ? p[idx(pb2s, x, y, z + 1)] : pc};
+                    for (int a = 0; a < 3; ++a) {
+                        if (cb[a][bid] < 2) continue;
+                        float &f = ch[a][idx(cb[a], x, y, z)];
+                        const float g = pn[a] - pc;
+                        const float t = scale * g;
+                        f = f - t;
+                    }
+                }
+    }
+}

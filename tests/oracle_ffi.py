@@ -88,6 +88,8 @@ def load():
     _sig(lib, "orc_surface_place", i64, f32p, i64, f32p, f32, i64, i64, i64, i64, f32, f32p, i64p, i64)
     _sig(lib, "orc_density_finalize", None, i32p, u16p, i64p, i64, f32, f32)
     _sig(lib, "orc_block_topology", i64, i64, i64, i64, u8p, i32, vp, u16p, i64p, i64p)
+    _sig(lib, "orc_mac_subtract_gradient", None, i64, i64, i64, f32p, i32p, f32p, i32p, f32p, i32p, f32p, i32p, i64p,
+         i64, f32)
     _LIB = Oracle(lib)
     return _LIB
 
