@@ -321,6 +321,50 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
     return out
 
 
+def surface_leg(pkg, lib, res, n_base, n_inst, reps=3):
+    """The surface-reconstruction pipeline (particles/mod.rs:220-302) at the
+    reference's surface_bench "dell big" size: a synthetic n_base-point cloud
+    (ellipsoid surface with 3% radial noise, LCG-seeded; the reference reads a
+    PLY of the same point count), res^3, n_inst instances, scale factor 0.01:
+    placement, min-splat, finalize, 5 mean-curvature sweeps, each timed with
+    HIP events (best of reps)."""
+    from msbg_rs_amd import scenes, surface as S
+    from msbg_rs_amd.particles import DensityGrid
+    u = scenes.lcg_uniform(31, 3 * n_base).reshape(n_base, 3).astype(np.float64)
+    th, ph = 2.0 * np.pi * u[:, 0], np.arccos(1.0 - 2.0 * u[:, 1])
+    rad = 1.0 + 0.03 * (2.0 * u[:, 2] - 1.0)
+    base = np.stack([0.8 * np.sin(ph) * np.cos(th), 1.0 * np.sin(ph) * np.sin(th), 0.6 * np.cos(ph)], 1) * rad[:, None]
+    base = base.astype(np.float32)
+    cfg = S.SurfaceConfig(res, res, res, n_instances=n_inst, instance_scale_factor=0.01)
+    bmin, span = S.bounding_box(base)
+    e = [lib.event() for _ in range(5)]
+    rec = lambda i: lib.call("msbg_event_record", e[i])
+    best = None
+    for _ in range(reps):
+        rec(0)
+        placed = S.place(base, bmin, span, cfg, lib=lib)
+        rec(1)
+        grid = DensityGrid(res, res, res, capacity_blocks=max(placed.active.size, 1), lib=lib)
+        k = grid.splat(placed.positions, cfg.r_scan())
+        rec(2)
+        S.finalize(grid, placed.active, cfg)
+        rec(3)
+        a = np.ascontiguousarray(placed.active, np.int64)
+        lib.call("msbg_sweep_density", grid.layout.h, grid.slab.ptr, a.ctypes.data, a.size, 1, cfg.smooth_dt,
+                 cfg.n_smooth_iters, 1)
+        rec(4)
+        lib.synchronize()
+        ms = {nm: lib.elapsed_ms(e[i], e[i + 1]) for i, nm in enumerate(("place", "splat", "finalize", "smooth"))}
+        ms["total"] = lib.elapsed_ms(e[0], e[4])
+        if best is None or ms["total"] < best["total"]:
+            best = ms
+    n = placed.positions.shape[0]
+    return {"res": res, "base_points": n_base, "instances": n_inst, "particles": int(n), "active_blocks": int(k),
+            "ms": best, "splat_particles_per_s": n / (best["splat"] * 1e-3),
+            "note": "place includes the footprint active-block set; splat includes the block allocation, the fill "
+                    "and the bucketing; smooth = 5 Density mean-curvature sweeps, clamped to [0, 1]"}
+
+
 SOLVE_VARIANTS = (("f32_plain", 0, False, False), ("f32_learned", 0, False, True), ("f16_refine", 1, True, False),
                   ("bf16_refine", 2, True, False), ("f16_refine_learned", 1, True, True),
                   ("bf16_refine_learned", 2, True, True))
@@ -400,6 +444,8 @@ def main():
                     help="domain edge of the HBM-bound scale-up sweep leg (0 = skip)")
     ap.add_argument("--solve-scale", type=int, default=1024, metavar="S",
                     help="domain edge of the HBM-bound Poisson solve leg (0 = skip)")
+    ap.add_argument("--surface", type=int, default=512, metavar="RES",
+                    help="resolution of the surface-reconstruction leg (64 instances of 8171 points; 0 = skip)")
     ap.add_argument("--flip", type=int, default=1024, metavar="S",
                     help="domain edge of the FLIP-step leg (dam-break box + droplets; 0 = skip)")
     ap.add_argument("--flip-particles", type=int, default=64 << 20)
@@ -518,6 +564,9 @@ def main():
     if args.solve_scale > 0:
         log("HBM-bound solve leg ...")
         out["solve_scale"] = solve_scale_leg(pkg, lib, args.solve_scale, 1)
+    if args.surface > 0:
+        log("surface leg ...")
+        out["surface"] = surface_leg(pkg, lib, args.surface, 8171, 64)
     if args.flip > 0:
         log("FLIP-step leg ...")
         out["flip"] = flip_leg(pkg, lib, args.flip, args.flip_particles, args.flip_sweeps)

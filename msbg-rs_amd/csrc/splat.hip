@@ -1,14 +1,17 @@
 // Density min-splat: particles::splat (particles/splat.rs:92-319) for
 // D = Density (u16, channel.rs:303-375) into a u16 slab over a grid layout.
 //
-// Each work-group takes one block's particle bucket: it stages the block's
-// (16 + 2m)^3 window (m = ceil(r_scan)) in LDS as u32 cells initialised to
-// 65535, every (particle, footprint row) pair lowers the cells of its row with
-// an LDS atomic min of the quantised distance, then the window is committed to
-// the 3x3x3 blocks it covers with dst = min(dst, staging).  Buckets run in 8
-// colour passes (block-coordinate parity), so same-colour windows never
-// overlap (16 + 2m <= 32) and the commit needs no global atomics.  min is
-// associative and commutative, so the result does not depend on any order.
+// A block's (16 + 2m)^3 window (m = ceil(r_scan)) is split into kZSlabs
+// z-slabs, one work-group each: the work-group stages its slab in LDS as u32
+// cells initialised to 65535, walks the block's particle bucket in chunks
+// (positions and footprint ranges staged in LDS, keeping the particles whose
+// footprint reaches the slab), lowers every (particle, footprint row) pair's
+// cells with an LDS atomic min of the quantised distance, then commits its
+// slab to the blocks it covers with dst = min(dst, staging).  Buckets run in
+// 8 colour passes (block-coordinate parity), so same-colour windows never
+// overlap (16 + 2m <= 32) and, the slabs of one window being disjoint, the
+// commit needs no global atomics.  min is associative and commutative, so
+// the result does not depend on any order.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -25,20 +28,29 @@ struct SplatParams {
     float r, dsq_max, inv;
 };
 
-constexpr int kMaxMargin = 6;  // (16 + 12)^3 u32 = 87.8 KB of LDS
+constexpr int kMaxMargin = 6;   // window edge W <= 28
+constexpr int kZSlabs = 8;      // work-groups per window
+constexpr int kSlabPlanes = 4;  // ceil(28 / 8)
+constexpr int kChunk = 256;     // particles staged per pass (one per thread)
 
 __global__ __launch_bounds__(256) void density_splat_kernel(SplatParams P, const int32_t *__restrict__ b2s,
                                                             uint16_t *slab, const float *__restrict__ pos,
                                                             const int32_t *__restrict__ blocks,
                                                             const int32_t *__restrict__ starts, int color) {
-    extern __shared__ uint32_t st[];
+    __shared__ uint32_t st[28 * 28 * kSlabPlanes];
     __shared__ int rslot[27];
-    const int k = blockIdx.x;
+    __shared__ float s_p[kChunk][3];      // particle position - 0.5
+    __shared__ int16_t s_r[kChunk][6];    // footprint: ix1, ix2, iy1, iy2, iz1, iz2 (window coordinates, slab-clipped in z)
+    __shared__ int s_n;
+    const int k = blockIdx.x / kZSlabs, zs = blockIdx.x % kZSlabs;
     const int bid = blocks[k];
     const int bx = bid % P.nx, by = (bid / P.nx) % P.ny, bz = bid / (P.nx * P.ny);
     if (((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) != color) return;  // uniform per work-group
-    const int W = P.W, m = P.m, W3 = W * W * W;
-    for (int i = threadIdx.x; i < W3; i += blockDim.x) st[i] = 0xffffu;
+    const int W = P.W, m = P.m;
+    const int z0 = zs * W / kZSlabs, z1 = (zs + 1) * W / kZSlabs;  // window planes [z0, z1) of this work-group
+    const int cells = W * W * (z1 - z0);
+    if (cells <= 0) return;
+    for (int i = threadIdx.x; i < cells; i += blockDim.x) st[i] = 0xffffu;
     const int b0x = bx * 16, b0y = by * 16, b0z = bz * 16, hi = 16 + m - 1;
     if (threadIdx.x < 27) {
         // region (rx, ry, rz): skipped when any voxel lies outside the domain
@@ -54,39 +66,56 @@ __global__ __launch_bounds__(256) void density_splat_kernel(SplatParams P, const
         }
         rslot[t] = slot;
     }
-    __syncthreads();
     // stage (splat.rs:179-225): footprint ceil(p - r - 0.5) .. floor(p + r - 0.5),
     // clipped to the window; d2 = dx*dx + (dz*dz + dy*dy)
     const int p0 = starts[k], p1 = starts[k + 1];
-    const int side = 2 * m + 1, rows = side * side;
-    const int64_t items = (int64_t)(p1 - p0) * rows;
-    for (int64_t i = threadIdx.x; i < items; i += blockDim.x) {
-        const int j = p0 + (int)(i / rows), rr = (int)(i % rows);
-        const float px = pos[3 * j], py = pos[3 * j + 1], pz = pos[3 * j + 2];
-        const int iy1 = min(max((int)ceilf((py - P.r) - 0.5f), b0y - m), b0y + hi);
-        const int iy2 = min(max((int)floorf((py + P.r) - 0.5f), b0y - m), b0y + hi);
-        const int iz1 = min(max((int)ceilf((pz - P.r) - 0.5f), b0z - m), b0z + hi);
-        const int iz2 = min(max((int)floorf((pz + P.r) - 0.5f), b0z - m), b0z + hi);
-        const int iy = iy1 + rr % side, iz = iz1 + rr / side;
-        if (iy > iy2 || iz > iz2) continue;
-        const int ix1 = min(max((int)ceilf((px - P.r) - 0.5f), b0x - m), b0x + hi);
-        const int ix2 = min(max((int)floorf((px + P.r) - 0.5f), b0x - m), b0x + hi);
-        const float dz = (float)iz - (pz - 0.5f), dy = (float)iy - (py - 0.5f);
-        const float dyz = dz * dz + dy * dy;
-        const float px0 = px - 0.5f;
-        uint32_t *row = st + ((iz - b0z + m) * W + (iy - b0y + m)) * W + (m - b0x);
-        for (int ix = ix1; ix <= ix2; ++ix) {
-            const float dx = (float)ix - px0;
-            const float d2 = dx * dx + dyz;
-            if (d2 <= P.dsq_max) atomicMin(row + ix, (uint32_t)(int)roundf((d2 * P.inv) * 65535.0f));
+    const int side = 2 * m + 1;
+    for (int c0 = p0; c0 < p1; c0 += kChunk) {
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        const int j = c0 + (int)threadIdx.x;
+        if (j < p1) {
+            const float px = pos[3 * j], py = pos[3 * j + 1], pz = pos[3 * j + 2];
+            const int iz1 = max(min(max((int)ceilf((pz - P.r) - 0.5f), b0z - m), b0z + hi) - (b0z - m), z0);
+            const int iz2 = min(min(max((int)floorf((pz + P.r) - 0.5f), b0z - m), b0z + hi) - (b0z - m), z1 - 1);
+            if (iz1 <= iz2) {
+                const int q = atomicAdd(&s_n, 1);
+                s_p[q][0] = px - 0.5f;
+                s_p[q][1] = py - 0.5f;
+                s_p[q][2] = pz - 0.5f;
+                s_r[q][0] = (int16_t)(min(max((int)ceilf((px - P.r) - 0.5f), b0x - m), b0x + hi) - (b0x - m));
+                s_r[q][1] = (int16_t)(min(max((int)floorf((px + P.r) - 0.5f), b0x - m), b0x + hi) - (b0x - m));
+                s_r[q][2] = (int16_t)(min(max((int)ceilf((py - P.r) - 0.5f), b0y - m), b0y + hi) - (b0y - m));
+                s_r[q][3] = (int16_t)(min(max((int)floorf((py + P.r) - 0.5f), b0y - m), b0y + hi) - (b0y - m));
+                s_r[q][4] = (int16_t)iz1;
+                s_r[q][5] = (int16_t)iz2;
+            }
         }
+        __syncthreads();
+        const int n = s_n, rows = kSlabPlanes * side;
+        for (int i = threadIdx.x; i < n * rows; i += blockDim.x) {
+            const int q = i / rows, rr = i - q * rows;
+            const int cz = s_r[q][4] + rr / side, cy = s_r[q][2] + rr % side;
+            if (cz > s_r[q][5] || cy > s_r[q][3]) continue;
+            // window cell (cx, cy, cz) is grid voxel (b0 - m) + c
+            const float dz = (float)(b0z - m + cz) - s_p[q][2], dy = (float)(b0y - m + cy) - s_p[q][1];
+            const float dyz = dz * dz + dy * dy;
+            uint32_t *row = st + ((cz - z0) * W + cy) * W;
+            const int cx2 = s_r[q][1];
+            for (int cx = s_r[q][0]; cx <= cx2; ++cx) {
+                const float dx = (float)(b0x - m + cx) - s_p[q][0];
+                const float d2 = dx * dx + dyz;
+                if (d2 <= P.dsq_max) atomicMin(row + cx, (uint32_t)(int)roundf((d2 * P.inv) * 65535.0f));
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    __syncthreads();  // the staging and the region table are complete even for an empty bucket
     // commit (splat.rs:227-313): each touched window cell, dst = min(dst, staged)
-    for (int c = threadIdx.x; c < W3; c += blockDim.x) {
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
         const uint32_t v = st[c];
         if (v == 0xffffu) continue;
-        const int cx = c % W, cy = (c / W) % W, cz = c / (W * W);
+        const int cx = c % W, cy = (c / W) % W, cz = z0 + c / (W * W);
         const int rx = cx < m ? 0 : (cx < m + 16 ? 1 : 2), ry = cy < m ? 0 : (cy < m + 16 ? 1 : 2),
                   rz = cz < m ? 0 : (cz < m + 16 ? 1 : 2);
         const int slot = rslot[rx + 3 * ry + 9 * rz];
@@ -179,12 +208,10 @@ int msbg_density_splat(msbg_grid *g, uint16_t *d_slab16, int64_t slab_slots, con
     P.r = r_scan;
     P.dsq_max = r_scan * r_scan;
     P.inv = 1.0f / P.dsq_max;
-    const size_t lds = sizeof(uint32_t) * P.W * P.W * P.W;
-    if (lds > 65536)
-        MSBG_HIP(hipFuncSetAttribute((const void *)density_splat_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
+    MSBG_CHECK(nb * kZSlabs < ((int64_t)1 << 31), MSBG_EINVAL, "density splat: too many occupied blocks");
     for (int color = 0; color < 8 && nb; ++color)
-        density_splat_kernel<<<(int)nb, 256, lds, s>>>(P, g->d_b2s, d_slab16, d_sorted, d_blocks, d_starts, color);
+        density_splat_kernel<<<(int)(nb * kZSlabs), 256, 0, s>>>(P, g->d_b2s, d_slab16, d_sorted, d_blocks, d_starts,
+                                                                color);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(s));
     return MSBG_OK;
