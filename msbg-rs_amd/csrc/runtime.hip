@@ -4,6 +4,7 @@
 #include <string>
 
 #include "internal.hpp"
+#include "primitives.hpp"
 
 namespace msbg {
 
@@ -35,6 +36,17 @@ int fill_f32(float *p, int64_t n, float v) {
     hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, g_stream, p, n, v);
     MSBG_HIP(hipGetLastError());
     return MSBG_OK;
+}
+
+// Block payload transfer: item i of a batch is block slots[i / N], cell i % N.
+__global__ void scatter_blocks_kernel(float *slab, const int32_t *slots, const float *src, int64_t n, int64_t N) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * N; i += (int64_t)gridDim.x * blockDim.x)
+        slab[(int64_t)slots[i / N] * N + i % N] = src[i];
+}
+
+__global__ void gather_blocks_kernel(const float *slab, const int32_t *slots, float *dst, int64_t n, int64_t N) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * N; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = slab[(int64_t)slots[i / N] * N + i % N];
 }
 
 // Sets every payload of `slots` (device list) to v.
@@ -315,11 +327,21 @@ int msbg_grid_write_blocks(msbg_grid *g, const int64_t *bids, int64_t n, const f
     int rc = check_bids(g, bids, n, "msbg_grid_write_blocks");
     if (rc) return rc;
     MSBG_CHECK(n == 0 || host, MSBG_EINVAL, "msbg_grid_write_blocks: null data");
-    for (int64_t i = 0; i < n; ++i)
+    std::vector<int32_t> slots((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
         MSBG_CHECK(g->b2s_host[bids[i]] >= 2, MSBG_EINVAL, "msbg_grid_write_blocks: block is not allocated");
-    for (int64_t i = 0; i < n; ++i)
-        MSBG_HIP(hipMemcpyAsync(g->d_slab + (int64_t)g->b2s_host[bids[i]] * g->N, host + i * g->N,
-                                sizeof(float) * g->N, hipMemcpyHostToDevice, g_stream));
+        slots[(size_t)i] = g->b2s_host[bids[i]];
+    }
+    if (n == 0) return MSBG_OK;
+    // one staged upload of the payload and the slot list, then a scatter kernel
+    float *stage = (float *)workspace(20, sizeof(float) * n * g->N + sizeof(int32_t) * n, &rc);
+    if (rc) return rc;
+    int32_t *d_slots = (int32_t *)(stage + n * g->N);
+    MSBG_HIP(hipMemcpyAsync(stage, host, sizeof(float) * n * g->N, hipMemcpyHostToDevice, g_stream));
+    MSBG_HIP(hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream));
+    hipLaunchKernelGGL(scatter_blocks_kernel, dim3((unsigned)std::min<int64_t>((n * g->N + 255) / 256, 65536)),
+                       dim3(256), 0, g_stream, g->d_slab, d_slots, stage, n, g->N);
+    MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(g_stream));
     return MSBG_OK;
 }
@@ -328,9 +350,18 @@ int msbg_grid_read_blocks(const msbg_grid *g, const int64_t *bids, int64_t n, fl
     int rc = check_bids(g, bids, n, "msbg_grid_read_blocks");
     if (rc) return rc;
     MSBG_CHECK(n == 0 || host, MSBG_EINVAL, "msbg_grid_read_blocks: null out");
-    for (int64_t i = 0; i < n; ++i)
-        MSBG_HIP(hipMemcpyAsync(host + i * g->N, g->d_slab + (int64_t)g->b2s_host[bids[i]] * g->N,
-                                sizeof(float) * g->N, hipMemcpyDeviceToHost, g_stream));
+    if (n == 0) return MSBG_OK;
+    std::vector<int32_t> slots((size_t)n);
+    for (int64_t i = 0; i < n; ++i) slots[(size_t)i] = g->b2s_host[bids[i]];
+    // a gather kernel into staging, then one download
+    float *stage = (float *)workspace(20, sizeof(float) * n * g->N + sizeof(int32_t) * n, &rc);
+    if (rc) return rc;
+    int32_t *d_slots = (int32_t *)(stage + n * g->N);
+    MSBG_HIP(hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_stream));
+    hipLaunchKernelGGL(gather_blocks_kernel, dim3((unsigned)std::min<int64_t>((n * g->N + 255) / 256, 65536)),
+                       dim3(256), 0, g_stream, g->d_slab, d_slots, stage, n, g->N);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipMemcpyAsync(host, stage, sizeof(float) * n * g->N, hipMemcpyDeviceToHost, g_stream));
     MSBG_HIP(hipStreamSynchronize(g_stream));
     return MSBG_OK;
 }
