@@ -342,10 +342,17 @@ typedef struct {
     int refine;           /* 0: red-black sweeps on x itself (storage must be F32);
                              1: FP32 iterative refinement: r = b - A x (FP32), rt = r / 2^k
                                 (2^k ~ rms(r)), e = 0, inner sweeps on A e = rt in `storage`,
-                                x = fmaf(2^k, e, x) */
+                                x = fmaf(2^k, e, x);
+                             2: the same refinement step with one multigrid V-cycle as the inner
+                                solve: block-aligned cell-centred levels (16^3, 8^3, 4^3, 2^3 cells
+                                per block, one per block, then the block lattice coarsened by 2),
+                                red-black smoothing (`storage` on the band, FP32 below), trilinear
+                                prolongation, FP64 conjugate gradients on the coarsest level
+                                (<= 1024 cells); sweeps_per_check = smoothing sweeps before and
+                                after the coarse correction on every level */
     float omega;          /* SOR factor, 0 < omega < 2 */
     int sweeps_per_check; /* red-black sweeps between FP64 residual checks (inner sweeps per refinement step) */
-    int max_sweeps;
+    int max_sweeps;       /* cap on the band's red-black sweeps (refine = 2: 2 sweeps_per_check per cycle) */
     int reserved;         /* 0 */
     double tol;           /* stop when ||b - A x||_2 <= tol * ||b||_2 (FP64 norms) */
     msbg_learned *learned;   /* NULL, or a network proposing a correction of x after every check period */
@@ -357,7 +364,7 @@ typedef struct {
     int converged;        /* 1 when the tolerance was met */
     int corrections;      /* learned corrections tried */
     int accepted;         /* ... and accepted (the true residual decreased) */
-    int reserved;
+    int levels;           /* refine = 2: multigrid levels including the band; else 0 */
     double rel_residual;  /* ||b - A x|| / ||b|| at the last check */
     double r2, b2;        /* squared norms at the last check */
     float ms;             /* wall time of the solve on the library stream (HIP events) */

@@ -6,6 +6,12 @@
 
 namespace msbg {
 
+using i64 = int64_t;
+
+namespace mg {
+struct Hierarchy;
+}
+
 // One listed value block: its slot, voxel origin and the slots of its six
 // face neighbours (-x, +x, -y, +y, -z, +z; unused where the face is on the
 // domain boundary).
@@ -16,6 +22,67 @@ struct PBlock {
 
 struct Dom {
     int32_t sx, sy, sz;
+};
+
+// Element access of the three storage types: ld widens to FP32, cv narrows
+// with round-to-nearest-even.
+template <int ST>
+struct Sto;
+template <>
+struct Sto<MSBG_STORAGE_F32> {
+    using T = float;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return p[i]; }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {  // i: multiple of 4
+        const float4 q = *reinterpret_cast<const float4 *>(p + i);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+    }
+    static __device__ __forceinline__ T cv(float v) { return v; }
+};
+template <>
+struct Sto<MSBG_STORAGE_F16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
+    static __device__ __forceinline__ float lo(uint32_t w) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+    }
+    static __device__ __forceinline__ float hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
+        v[0] = lo(q.x);
+        v[1] = hi(q.x);
+        v[2] = lo(q.y);
+        v[3] = hi(q.y);
+    }
+    // The FP32 value is materialised before the RNE narrowing: without the
+    // register barrier the compiler fuses fmaf + cvt into one mixed-precision
+    // FMA (v_fma_mixlo_f16), which rounds once to f16 instead of f32 then f16.
+    static __device__ __forceinline__ T cv(float v) {
+        asm volatile("" : "+v"(v));
+        return __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+};
+template <>
+struct Sto<MSBG_STORAGE_BF16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T *p, i64 i) { return __uint_as_float((uint32_t)p[i] << 16); }
+    static __device__ __forceinline__ float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    static __device__ __forceinline__ float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
+        v[0] = lo(q.x);
+        v[1] = hi(q.x);
+        v[2] = lo(q.y);
+        v[3] = hi(q.y);
+    }
+    static __device__ __forceinline__ T cv(float v) {
+        uint32_t u = __float_as_uint(v);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (T)(u >> 16);
+    }
 };
 
 // Per-block kernels of 256 threads; the cell-wise ones (correction, dots) take
@@ -32,6 +99,8 @@ struct msbg_poisson {
     int n_wg = 0;                     // work-groups of the kSlices-per-block kernels
     int zcol[3] = {1, 1, 1};          // column depth of the relaxation kernels, per storage
     msbg::PBlock *d_blocks = nullptr;
+    std::vector<int64_t> bids;       // the listed block ids, in list order
+    msbg::mg::Hierarchy *mg = nullptr;  // multigrid levels, built by the first multigrid solve
     int32_t *d_b2s = nullptr;        // copy of the layout grid's bid -> slot table
     uint8_t *d_listed = nullptr;     // per block: 1 if listed
     double2 *d_partial = nullptr;
@@ -66,8 +135,26 @@ namespace pband {
 void residual(msbg_poisson *p, hipStream_t s, int storage, const void *x, const void *b, float *r);
 // Copies p->d_norm to the host (synchronizes the library stream).
 int read_norms(msbg_poisson *p, double *r2, double *b2);
+// r = b - A x over the listed cells (FP32), without the norm reduction.
+void residual_field(msbg_poisson *p, hipStream_t s, int storage, const void *x, const void *b, float *r);
+// `sweeps` red-black sweeps in place, colour 0 then 1 (reverse: 1 then 0).
+void rb_sweeps(msbg_poisson *p, hipStream_t s, int storage, void *x, const void *b, float omega, int sweeps,
+               bool reverse);
 
 }  // namespace pband
+
+namespace mg {
+
+// Builds the coarse levels of p's band (once; host tables + device fields).
+int ensure(msbg_poisson *p);
+void release(msbg_poisson *p);
+// Levels including the band itself (0 before ensure).
+int n_levels(const msbg_poisson *p);
+// One V-cycle on A e = rt (p->d_e, p->d_rt in `storage`, e = 0 on entry):
+// nu pre- and post-smoothing red-black sweeps per level; p->d_r is scratch.
+void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu);
+
+}  // namespace mg
 
 namespace learned {
 

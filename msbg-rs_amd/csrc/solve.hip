@@ -26,65 +26,6 @@ namespace {
 
 using i64 = int64_t;
 
-template <int ST>
-struct Sto;
-template <>
-struct Sto<MSBG_STORAGE_F32> {
-    using T = float;
-    static __device__ __forceinline__ float ld(const T *p, i64 i) { return p[i]; }
-    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {  // i: multiple of 4
-        const float4 q = *reinterpret_cast<const float4 *>(p + i);
-        v[0] = q.x;
-        v[1] = q.y;
-        v[2] = q.z;
-        v[3] = q.w;
-    }
-    static __device__ __forceinline__ T cv(float v) { return v; }
-};
-template <>
-struct Sto<MSBG_STORAGE_F16> {
-    using T = uint16_t;
-    static __device__ __forceinline__ float ld(const T *p, i64 i) { return (float)__builtin_bit_cast(_Float16, p[i]); }
-    static __device__ __forceinline__ float lo(uint32_t w) {
-        return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
-    }
-    static __device__ __forceinline__ float hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
-    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
-        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
-        v[0] = lo(q.x);
-        v[1] = hi(q.x);
-        v[2] = lo(q.y);
-        v[3] = hi(q.y);
-    }
-    // The FP32 value is materialised before the RNE narrowing: without the
-    // register barrier the compiler fuses fmaf + cvt into one mixed-precision
-    // FMA (v_fma_mixlo_f16), which rounds once to f16 instead of f32 then f16.
-    static __device__ __forceinline__ T cv(float v) {
-        asm volatile("" : "+v"(v));
-        return __builtin_bit_cast(uint16_t, (_Float16)v);
-    }
-};
-template <>
-struct Sto<MSBG_STORAGE_BF16> {
-    using T = uint16_t;
-    static __device__ __forceinline__ float ld(const T *p, i64 i) { return __uint_as_float((uint32_t)p[i] << 16); }
-    static __device__ __forceinline__ float lo(uint32_t w) { return __uint_as_float(w << 16); }
-    static __device__ __forceinline__ float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
-    static __device__ __forceinline__ void ld4(const T *p, i64 i, float *v) {
-        const uint2 q = *reinterpret_cast<const uint2 *>(p + i);
-        v[0] = lo(q.x);
-        v[1] = hi(q.x);
-        v[2] = lo(q.y);
-        v[3] = hi(q.y);
-    }
-    static __device__ __forceinline__ T cv(float v) {
-        uint32_t u = __float_as_uint(v);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (T)(u >> 16);
-    }
-};
-
 int elem_bytes(int storage) { return storage == MSBG_STORAGE_F32 ? 4 : 2; }
 
 // One row vector of W consecutive cells as stored (W = 4 for FP32, 8 for the
@@ -350,11 +291,12 @@ __global__ __launch_bounds__(256) void reduce_norms_kernel(const double2 *__rest
     }
 }
 
-// Refinement step start: rt = T(r / scale), e = 0 on the listed cells
+// Refinement step start: rt = T(r / scale * extra), e = 0 on the listed cells
+// (extra: an exact power of two)
 // (column mapping of the relaxation, row vectors where the row is whole).
 template <int ST, int Z>
 __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__restrict__ blocks, Dom d, const float *r,
-                                                            const float *scale, typename Sto<ST>::T *rt,
+                                                            const float *scale, float extra, typename Sto<ST>::T *rt,
                                                             typename Sto<ST>::T *e) {
     constexpr int W = ColMap<ST, Z>::W;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -362,7 +304,7 @@ __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__rest
     Col<ST, Z> Q;
     int vid;
     if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
-    const float inv = scale[1];
+    const float inv = scale[1] * extra;
 #pragma unroll
     for (int j = 0; j < Z; ++j) {
         if (j >= Q.nz) break;
@@ -390,18 +332,18 @@ __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__rest
     }
 }
 
-// Refinement step end: x = fmaf(scale, e, x) on the listed cells.
+// Refinement step end: x = fmaf(scale / extra, e, x) on the listed cells.
 template <int ST, int Z>
 __global__ __launch_bounds__(kThreads) void ir_update_kernel(const PBlock *__restrict__ blocks, Dom d,
                                                              const typename Sto<ST>::T *e, const float *scale,
-                                                             float *x) {
+                                                             float extra, float *x) {
     constexpr int W = ColMap<ST, Z>::W;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
     Col<ST, Z> Q;
     int vid;
     if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
-    const float s = scale[0];
+    const float s = scale[0] / extra;
 #pragma unroll
     for (int j = 0; j < Z; ++j) {
         if (j >= Q.nz) break;
@@ -544,48 +486,45 @@ void relax(msbg_poisson *p, hipStream_t s, int st, const void *xin, void *xout, 
 }
 
 void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, float omega, int sweeps) {
-    for (int i = 0; i < sweeps; ++i) {
-        relax(p, s, st, x, x, b, omega, 0, true);
-        relax(p, s, st, x, x, b, omega, 1, true);
-    }
+    pband::rb_sweeps(p, s, st, x, b, omega, sweeps, false);
 }
 
 template <int ST>
-void ir_scale_st(msbg_poisson *p, hipStream_t s) {
+void ir_scale_st(msbg_poisson *p, hipStream_t s, float extra) {
     using T = typename Sto<ST>::T;
     with_z<ST>(p, [&](auto zc) {
         constexpr int Z = decltype(zc)::value;
-        ir_scale_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale,
+        ir_scale_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale, extra,
                                                                           (T *)p->d_rt, (T *)p->d_e);
     });
 }
 
 template <int ST>
-void ir_update_st(msbg_poisson *p, hipStream_t s, float *x) {
+void ir_update_st(msbg_poisson *p, hipStream_t s, float *x, float extra) {
     using T = typename Sto<ST>::T;
     with_z<ST>(p, [&](auto zc) {
         constexpr int Z = decltype(zc)::value;
         ir_update_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)p->d_e,
-                                                                           p->d_scale, x);
+                                                                           p->d_scale, extra, x);
     });
 }
 
-void ir_scale(msbg_poisson *p, hipStream_t s, int st) {
+void ir_scale(msbg_poisson *p, hipStream_t s, int st, float extra) {
     if (st == MSBG_STORAGE_F32)
-        ir_scale_st<MSBG_STORAGE_F32>(p, s);
+        ir_scale_st<MSBG_STORAGE_F32>(p, s, extra);
     else if (st == MSBG_STORAGE_F16)
-        ir_scale_st<MSBG_STORAGE_F16>(p, s);
+        ir_scale_st<MSBG_STORAGE_F16>(p, s, extra);
     else
-        ir_scale_st<MSBG_STORAGE_BF16>(p, s);
+        ir_scale_st<MSBG_STORAGE_BF16>(p, s, extra);
 }
 
-void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x) {
+void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x, float extra) {
     if (st == MSBG_STORAGE_F32)
-        ir_update_st<MSBG_STORAGE_F32>(p, s, x);
+        ir_update_st<MSBG_STORAGE_F32>(p, s, x, extra);
     else if (st == MSBG_STORAGE_F16)
-        ir_update_st<MSBG_STORAGE_F16>(p, s, x);
+        ir_update_st<MSBG_STORAGE_F16>(p, s, x, extra);
     else
-        ir_update_st<MSBG_STORAGE_BF16>(p, s, x);
+        ir_update_st<MSBG_STORAGE_BF16>(p, s, x, extra);
 }
 
 // Inner fields for `storage`; slots outside the listed blocks stay zero (the
@@ -617,16 +556,26 @@ int ensure_inner(msbg_poisson *p, int storage) {
 
 // One check period.  mode 0: `sweeps` red-black sweeps on x (FP32), then the
 // residual.  mode 1: refinement step rt = r / s, e = 0, `sweeps` inner sweeps
-// on e in `storage`, x += s e, then the residual (into r).
+// on e in `storage`, x += s e, then the residual (into r).  mode 2: the same
+// refinement step with one multigrid V-cycle (`sweeps` pre- and
+// post-smoothing sweeps per level) as the inner solve.
 void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, const float *b, float omega,
             int sweeps) {
     if (mode == 0) {
         rb_sweeps(p, s, MSBG_STORAGE_F32, x, b, omega, sweeps);
         residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
     } else {
-        ir_scale(p, s, storage);
-        rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps);
-        ir_update(p, s, storage, x);
+        // A V-cycle returns the whole correction, |e| up to ~1 / lambda_min
+        // times rms(rt): the FP16 inner problem is scaled down by 2^-12 more,
+        // leaving 2^12 * 65504 of headroom (entries below rms / 4 become
+        // subnormal, an absolute error <= 2^-13 rms).
+        const float extra = (mode == 2 && storage == MSBG_STORAGE_F16) ? 0x1p-12f : 1.0f;
+        ir_scale(p, s, storage, extra);
+        if (mode == 2)
+            mg::vcycle(p, s, storage, omega, sweeps);
+        else
+            rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps);
+        ir_update(p, s, storage, x, extra);
         residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
     }
 }
@@ -656,6 +605,39 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
 }
 
 }  // namespace
+}  // namespace msbg
+
+namespace msbg {
+namespace pband {
+
+template <int ST>
+void residual_field_st(msbg_poisson *p, hipStream_t s, const void *x, const void *b, float *r) {
+    using T = typename Sto<ST>::T;
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        residual_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, (const T *)x,
+                                                                          (const T *)b, r, p->d_partial);
+    });
+}
+
+void residual_field(msbg_poisson *p, hipStream_t s, int st, const void *x, const void *b, float *r) {
+    if (st == MSBG_STORAGE_F32)
+        residual_field_st<MSBG_STORAGE_F32>(p, s, x, b, r);
+    else if (st == MSBG_STORAGE_F16)
+        residual_field_st<MSBG_STORAGE_F16>(p, s, x, b, r);
+    else
+        residual_field_st<MSBG_STORAGE_BF16>(p, s, x, b, r);
+}
+
+void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, float omega, int sweeps,
+               bool reverse) {
+    for (int i = 0; i < sweeps; ++i) {
+        relax(p, s, st, x, x, b, omega, reverse ? 1 : 0, true);
+        relax(p, s, st, x, x, b, omega, reverse ? 0 : 1, true);
+    }
+}
+
+}  // namespace pband
 }  // namespace msbg
 
 extern "C" {
@@ -691,6 +673,7 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
         }
     }
     msbg_poisson *p = new msbg_poisson();
+    p->bids.assign(active, active + n_active);
     p->dom = Dom{(int32_t)layout->sx, (int32_t)layout->sy, (int32_t)layout->sz};
     p->n_slots = layout->n_slots;
     p->n_active = n_active;
@@ -736,6 +719,7 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
 int msbg_poisson_free(msbg_poisson *p) {
     if (!p) return MSBG_OK;
     drop_graph(p);
+    mg::release(p);
     if (p->jexec) hipGraphExecDestroy(p->jexec);
     if (p->cap_stream) hipStreamDestroy(p->cap_stream);
     hipFree(p->d_blocks);
@@ -858,7 +842,7 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
     MSBG_CHECK(p && prm && x && b && st, MSBG_EINVAL, "null argument");
     int rc = check_storage(prm->storage);
     if (rc) return rc;
-    MSBG_CHECK(prm->refine == 0 || prm->refine == 1, MSBG_EINVAL, "refine must be 0 or 1");
+    MSBG_CHECK(prm->refine >= 0 && prm->refine <= 2, MSBG_EINVAL, "refine must be 0, 1 or 2");
     MSBG_CHECK(prm->refine || prm->storage == MSBG_STORAGE_F32, MSBG_EINVAL,
                "sweeps on x itself need FP32 storage (use refine = 1 for FP16 / BF16)");
     MSBG_CHECK(prm->omega > 0.0f && prm->omega < 2.0f, MSBG_EINVAL, "omega must be in (0, 2)");
@@ -878,9 +862,14 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
     if (mode) {
         rc = ensure_inner(p, prm->storage);
         if (rc) return rc;
+        if (mode == 2) {
+            rc = mg::ensure(p);
+            if (rc) return rc;
+        }
     } else if (!p->d_r) {
         MSBG_HIP(hipMalloc(&p->d_r, (size_t)p->n_slots * 4096 * 4));
     }
+    st->levels = mode == 2 ? mg::n_levels(p) : 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MSBG_HIP(hipEventCreate(&e0));
     MSBG_HIP(hipEventCreate(&e1));
@@ -912,10 +901,12 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
             st->converged = 1;
             break;
         }
-        if (!std::isfinite(rel) || st->sweeps + prm->sweeps_per_check > prm->max_sweeps) break;
+        // fine-level red-black sweeps of one check period
+        const int per = mode == 2 ? 2 * prm->sweeps_per_check : prm->sweeps_per_check;
+        if (!std::isfinite(rel) || st->sweeps + per > prm->max_sweeps) break;
         rc = run_period_graph(p, mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
         if (rc) break;
-        st->sweeps += prm->sweeps_per_check;
+        st->sweeps += per;
         st->checks += 1;
         rc = read_norms(p, &r2, &b2);
         corrected = false;

@@ -39,8 +39,9 @@ class FlipStep:
         self.capacity_blocks = capacity_blocks
 
     def run(self, pos, vel, mass, r_p=2.0, alpha=0.95, storage=Storage.F32, refine=False, omega=1.9,
-            sweeps_per_check=4, max_sweeps=20000, tol=1e-6, learned=None, phi=None):
-        """One step.  pos / vel / mass: device arrays (n x 3, n x 3, n).
+            sweeps_per_check=4, max_sweeps=20000, tol=1e-6, learned=None, phi=None, multigrid=False):
+        """One step.  pos / vel / mass: device arrays (n x 3, n x 3, n); the
+        solve's arguments are PoissonBand.solve's.
         Returns (sorted positions, new velocities, stats) with stats holding
         the solve's msbg_poisson_stats, the active block count, the band's
         cell count and per-stage ms (HIP events)."""
@@ -71,7 +72,7 @@ class FlipStep:
         rec(3)
         p = band.field(0)
         st = band.solve(p, b, storage=storage, refine=refine, omega=omega, sweeps_per_check=sweeps_per_check,
-                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi)
+                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi, multigrid=multigrid)
         rec(4)
         self.mac.subtract_gradient(layout, p, active, -1.0)
         rec(5)

@@ -117,7 +117,7 @@ class PoissonParams(C.Structure):
 class PoissonStats(C.Structure):
     """msbg_poisson_stats (include/msbg_hip.h)."""
     _fields_ = [("sweeps", C.c_int), ("checks", C.c_int), ("converged", C.c_int), ("corrections", C.c_int),
-                ("accepted", C.c_int), ("reserved", C.c_int), ("rel_residual", C.c_double), ("r2", C.c_double),
+                ("accepted", C.c_int), ("levels", C.c_int), ("rel_residual", C.c_double), ("r2", C.c_double),
                 ("b2", C.c_double), ("ms", C.c_float)]
 
 

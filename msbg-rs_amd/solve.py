@@ -87,17 +87,23 @@ class PoissonBand:
         return r2.value, b2.value
 
     def solve(self, x, b, storage=Storage.F32, refine=False, omega=1.9, sweeps_per_check=10, max_sweeps=10000,
-              tol=1e-6, learned=None, phi=None):
+              tol=1e-6, learned=None, phi=None, multigrid=False):
         """Red-black SOR solve of A x = b (x, b FP32 fields; x = initial guess,
         updated in place).  refine=True: FP32 iterative refinement with the
-        inner sweeps in `storage`.  learned (a CorrectorNet) with phi (the
-        phase-field DeviceGrid): a correction is tried after every check
-        period.  Returns the msbg_poisson_stats fields."""
-        prm = _lib.PoissonParams(storage, int(refine), omega, sweeps_per_check, max_sweeps, 0, tol,
+        inner sweeps in `storage`.  multigrid=True: FP32 iterative refinement
+        whose inner solve is one multigrid V-cycle per check, with
+        sweeps_per_check smoothing sweeps (factor omega; 1.0 = Gauss-Seidel)
+        before and after the coarse correction on every level, the band's in
+        `storage`; max_sweeps caps the band's sweeps (2 sweeps_per_check per
+        cycle).  learned (a CorrectorNet) with phi (the phase-field
+        DeviceGrid): a correction is tried after every check period.  Returns
+        the msbg_poisson_stats fields."""
+        mode = 2 if multigrid else int(refine)
+        prm = _lib.PoissonParams(storage, mode, omega, sweeps_per_check, max_sweeps, 0, tol,
                                  learned.h if learned is not None else None, phi.h if phi is not None else None)
         st = _lib.PoissonStats()
         self.lib.call("msbg_poisson_solve", self.h, C.byref(prm), x.ptr, b.ptr, C.byref(st))
-        return {k: getattr(st, k) for k, _ in _lib.PoissonStats._fields_ if k != "reserved"}
+        return {k: getattr(st, k) for k, _ in _lib.PoissonStats._fields_}
 
     def net_input(self, phi, r, inv_scale):
         """Network input of every listed block (device bf16 bits, n x 20^3 x 2)."""

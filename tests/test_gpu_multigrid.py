@@ -1,0 +1,140 @@
+"""GPU multigrid solves (msbg_poisson_solve with refine = 2, one V-cycle per
+FP32 refinement step), judged by the oracle:
+
+- the returned x's residual, recomputed by the oracle (FP64 norms), meets the
+  tolerance and agrees with the solve's own report;
+- on the 128^3 shell its error against the oracle's FP64 CG solution is
+  bounded (FP32, FP16 and BF16 band storage);
+- every V-cycle contracts the residual by a grid-independent factor (the
+  property that makes the method worth its cost), on bands whose coarsest
+  level is the one-cell-per-block level and on bands that need coarser
+  lattice levels;
+- solves are deterministic (no atomics: two solves from the same start give
+  the same bits);
+- the FLIP projection solved by multigrid leaves the divergence at the
+  solve's residual.
+"""
+import numpy as np
+import pytest
+
+from solve_helpers import band_problem, listed_mask, oracle_cg64, oracle_residual
+from transfer_helpers import dense_from_slabs
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(pkg, hip, g, active):
+    layout = pkg.DeviceGrid.from_host(g, lib=hip)
+    return layout, pkg.PoissonBand(layout, active, lib=hip)
+
+
+def check_residual(orc, g, active, xg, b, st, tol):
+    """The oracle's residual of the returned x meets tol and matches the report."""
+    _, r2, b2 = oracle_residual(orc, g, active, np.ascontiguousarray(xg), np.ascontiguousarray(b))
+    rel = np.sqrt(r2 / b2)
+    assert rel <= tol * (1 + 1e-9), (rel, st)
+    # FP64 norms summed in different orders
+    eps = max(1e-12, 2.0 * int(listed_mask(g, active).sum()) * 2.0 ** -53)
+    assert st["rel_residual"] == pytest.approx(rel, rel=eps)
+    return rel
+
+
+@pytest.mark.parametrize("storage,err_max", [(0, 1e-5), (1, 2e-5), (2, 2e-5)])
+def test_shell_multigrid_to_1e6_vs_fp64(pkg, orc, hip, storage, err_max):
+    from msbg_rs_amd import scenes
+    g, bids, _ = scenes.shell_scene()
+    _, pb = setup(pkg, hip, g, bids)
+    b = scenes.poisson_rhs(g, bids)
+    dx, db = pb.field(0), hip.to_device(b)
+    st = pb.solve(dx, db, storage=storage, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 20,
+                  tol=1e-6)
+    assert st["converged"] and st["levels"] >= 5, st
+    assert st["sweeps"] <= 4 * 12, st  # <= 12 V-cycles
+    xg = dx.get().reshape(-1, 4096)
+    check_residual(orc, g, bids, xg, b, st, 1e-6)
+    xd, _ = oracle_cg64(orc, g, bids, b.astype(np.float64), tol=1e-12)
+    m = listed_mask(g, bids)
+    s = (g.sx, g.sy, g.sz)
+    ref = dense_from_slabs(g.b2s, xd, s)[m]
+    err = np.abs(dense_from_slabs(g.b2s, xg, s)[m] - ref).max() / np.abs(ref).max()
+    assert err < err_max, err
+
+
+def cycle_residuals(pkg, hip, g, active, b, storage, cycles, x0=None):
+    """Relative residual after 1..cycles V-cycles (nu = 2), each from x0."""
+    _, pb = setup(pkg, hip, g, active)
+    db = hip.to_device(np.ascontiguousarray(b))
+    out = []
+    for k in range(1, cycles + 1):
+        dx = pb.field(0, host=x0)
+        st = pb.solve(dx, db, storage=storage, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * k,
+                      tol=0.0)
+        assert st["sweeps"] == 4 * k and not st["converged"]
+        out.append(st["rel_residual"])
+    print("levels %d, residual after 1..%d cycles: %s" % (st["levels"], cycles, ["%.2e" % r for r in out]))
+    return out, st["levels"]
+
+
+@pytest.mark.parametrize("s,seed,frac,min_levels,factor", [((40, 33, 48), 1, 0.7, 5, 0.35),
+                                                           ((96, 80, 64), 2, 0.5, 5, 0.35),
+                                                           ((250, 256, 244), 3, 0.9, 6, 0.7)])
+@pytest.mark.parametrize("storage", [0, 1])
+def test_vcycle_contracts_on_random_bands(pkg, hip, s, seed, frac, min_levels, factor, storage):
+    """Random value / absent / full block sets (ragged domains, nonzero
+    Dirichlet values in the absent / full / unlisted blocks): after the first
+    cycle every V-cycle shrinks the residual by `factor` or more, down to what
+    the storage resolves.  The 250 x 256 x 244 band has single-block holes
+    scattered through a 16^3-block lattice: its block-lattice levels (the
+    Galerkin aggregates of 2^3 blocks around the holes) correct less, and
+    its cycles contract by ~0.6 after the third, against ~0.13 on the
+    shell, the 512^3 band and the FLIP band."""
+    g, active, x, b = band_problem(pkg, *s, seed, frac)
+    rel, levels = cycle_residuals(pkg, hip, g, active, b, storage, 6, x0=x.reshape(-1))
+    assert levels >= min_levels
+    floor = 1e-6 if storage == 0 else 5e-4  # one FP16 refinement step resolves ~2^-11 of r
+    for a, c in zip(rel, rel[1:]):
+        if a > floor:
+            assert c <= factor * a, rel
+    assert rel[-1] < 1e-3, rel
+
+
+@pytest.mark.parametrize("s,seed,frac", [((40, 33, 48), 1, 0.7), ((250, 256, 244), 3, 0.9)])
+def test_random_band_solve_vs_oracle_residual(pkg, orc, hip, s, seed, frac):
+    g, active, x, b = band_problem(pkg, *s, seed, frac)
+    _, pb = setup(pkg, hip, g, active)
+    dx, db = pb.field(0, host=x.reshape(-1)), hip.to_device(np.ascontiguousarray(b))
+    st = pb.solve(dx, db, storage=1, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 30, tol=1e-5)
+    assert st["converged"], st
+    xg = dx.get().reshape(-1, 4096)
+    check_residual(orc, g, active, xg, b, st, 1e-5)
+    # cells outside the listed blocks are untouched
+    keep = np.ones(g.n_slots, bool)
+    keep[g.b2s[active]] = False
+    assert np.array_equal(xg[keep], x[keep])
+
+
+def test_multigrid_solve_is_deterministic(pkg, hip):
+    from msbg_rs_amd import scenes
+    g, bids, _ = scenes.shell_scene()
+    _, pb = setup(pkg, hip, g, bids)
+    db = hip.to_device(scenes.poisson_rhs(g, bids))
+    outs = []
+    for _ in range(2):
+        dx = pb.field(0)
+        pb.solve(dx, db, storage=1, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 6, tol=0.0)
+        outs.append(dx.get())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_flip_projection_by_multigrid(pkg, hip):
+    """The FLIP step's pressure solved by multigrid (FP16 band storage) to
+    2e-6 (this right-hand side has ||p|| / ||b|| ~ 10, and an FP32 pressure
+    cannot hold a relative residual below ~2.3e-7 ||p|| / ||b||): the
+    divergence drops to the residual (1e-4 of its L2 norm or less)."""
+    from test_gpu_flip import run_step
+    fs, ps, out, st = run_step(pkg, hip, 64, 200000, storage=1, refine=True, sweeps_per_check=2, max_sweeps=4 * 30,
+                               tol=2e-6, omega=1.0, multigrid=True)
+    assert st["solve"]["converged"] and st["solve"]["rel_residual"] <= 2e-6, st
+    assert st["solve"]["sweeps"] <= 4 * 15, st
+    n0, n1 = fs.divergence_norms()
+    assert n1 <= 1e-4 * n0, (n0, n1)
