@@ -257,19 +257,25 @@ def solve_scale_leg(pkg, lib, S, reps):
     b = lib.empty(band.n_slots * 4096, np.float32)
     lib.call("msbg_memcpy_dtod", b.ptr, g.views()[0], band.n_slots * 4096 * 4)
     out = {"domain": S, "blocks": int(bids.size), "cells": band.n_cells, "fp32_field_bytes": band.n_cells * 4}
-    for name, storage, refine in (("f32_plain", 0, False), ("f16_refine", 1, True), ("bf16_refine", 2, True)):
+    names = ("f32_plain", "f16_refine", "bf16_refine", "f32_mg", "f16_mg", "bf16_mg")
+    for name in names:
+        storage, method = {"f32": 0, "f16": 1, "bf16": 2}[name.split("_")[0]], name.split("_")[1]
+        method = {"plain": "sor"}.get(method, method)
+        args = solve_args(method)
+        if method != "mg":
+            args["sweeps_per_check"] = 10
         best = None
         for _ in range(reps):
             x = band.field(0)
-            st = band.solve(x, b, storage=storage, refine=refine, omega=1.9, sweeps_per_check=10, max_sweeps=20000,
-                            tol=1e-6)
+            st = band.solve(x, b, storage=storage, tol=1e-6, **args)
             x.free()
             if best is None or st["ms"] < best["ms"]:
                 best = st
-        out[name] = {k: best[k] for k in ("ms", "sweeps", "rel_residual", "converged")}
+        out[name] = {k: best[k] for k in ("ms", "sweeps", "rel_residual", "converged", "levels")}
         out[name]["ms_per_sweep"] = best["ms"] / max(best["sweeps"], 1)
+        log("solve_scale %s: %.2f ms, %d sweeps" % (name, best["ms"], best["sweeps"]))
     base = out["f32_plain"]["ms"]
-    for k in ("f32_plain", "f16_refine", "bf16_refine"):
+    for k in names:
         out[k]["time_vs_f32_plain"] = out[k]["ms"] / base
     b.free()
     band.close()
@@ -281,8 +287,9 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
     """One FLIP-style step (configuration C5's particles: dam-break box +
     droplets, S^3, n particles): Morton sort, MAC P2G, divergence, band
     pressure solve (target 1e-6, at most max_sweeps sweeps), projection,
-    staggered G2P with the FLIP/PIC blend.  The plain FP32 pipeline and the
-    FP16-storage refinement solve, each timed per stage with HIP events."""
+    staggered G2P with the FLIP/PIC blend.  The plain FP32 pipeline (SOR),
+    the FP16-storage refinement solve and the multigrid solves (FP32 and
+    FP16 band storage), each timed per stage with HIP events."""
     from msbg_rs_amd import scenes
     from msbg_rs_amd.flip import FlipStep
     from msbg_rs_amd.particles import active_blocks
@@ -294,11 +301,14 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
     cap = int(active_blocks(S, S, S, dp, 3.0, lib=lib).size * 1.1) + 64  # above the P2G footprint (r_p = 2)
     fs = FlipStep(S, S, S, capacity_blocks=cap, lib=lib)
     out = {"domain": S, "particles": n, "capacity_blocks": cap}
-    for name, storage, refine in (("f32_plain", 0, False), ("f16_refine", 1, True)):
+    for name, storage, method in (("f32_plain", 0, "sor"), ("f16_refine", 1, "refine"), ("f32_mg", 0, "mg"),
+                                  ("f16_mg", 1, "mg")):
+        args = solve_args(method)
+        if method != "mg":
+            args.update(sweeps_per_check=10, max_sweeps=max_sweeps)
         best = None
         for _ in range(reps):
-            _, _, st = fs.run(dp, dv, dm, r_p=2.0, alpha=0.95, storage=storage, refine=refine, omega=1.9,
-                              sweeps_per_check=10, max_sweeps=max_sweeps, tol=1e-6)
+            _, _, st = fs.run(dp, dv, dm, r_p=2.0, alpha=0.95, storage=storage, tol=1e-6, **args)
             if best is None or st["ms"]["total"] < best["ms"]["total"]:
                 best = st
         layout, band, p, b, active, band_blocks = fs.last
@@ -307,13 +317,15 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
         sv = best["solve"]
         out[name] = {"ms": best["ms"], "active_blocks": best["active_blocks"], "band_blocks": best["band_blocks"],
                      "cells": best["cells"],
-                     "sweeps": sv["sweeps"], "rel_residual": sv["rel_residual"], "converged": sv["converged"],
+                     "sweeps": sv["sweeps"], "levels": sv["levels"], "rel_residual": sv["rel_residual"],
+                     "converged": sv["converged"],
                      "p_over_b": float(np.sqrt(pn)) / max(d0, 1e-30), "div_before": d0, "div_after": d1,
                      "particles_per_s_step": n / (best["ms"]["total"] * 1e-3),
                      "particles_per_s_transfers": n / ((best["ms"]["sort"] + best["ms"]["p2g"] + best["ms"]["g2p"]) * 1e-3)}
         log("flip %s: %.1f ms total, %d sweeps, rel %.2e" % (name, best["ms"]["total"], sv["sweeps"],
                                                              sv["rel_residual"]))
-    out["f16_vs_f32_total"] = out["f16_refine"]["ms"]["total"] / out["f32_plain"]["ms"]["total"]
+    for k in ("f16_refine", "f32_mg", "f16_mg"):
+        out[k + "_vs_f32_total"] = out[k]["ms"]["total"] / out["f32_plain"]["ms"]["total"]
     out["note"] = ("single resolution level at S^3; the pressure band is the MAC blocks (those the particles' P2G "
                    "footprint touches) whose -x/-y/-z neighbours are MAC blocks, p = 0 outside it; div_before / "
                    "div_after are L2 norms over the band's cells; an FP32 pressure stalls near 2.3e-7 ||p||/||b||, "
@@ -365,28 +377,40 @@ def surface_leg(pkg, lib, res, n_base, n_inst, reps=3):
                     "and the bucketing; smooth = 5 Density mean-curvature sweeps, clamped to [0, 1]"}
 
 
-SOLVE_VARIANTS = (("f32_plain", 0, False, False), ("f32_learned", 0, False, True), ("f16_refine", 1, True, False),
-                  ("bf16_refine", 2, True, False), ("f16_refine_learned", 1, True, True),
-                  ("bf16_refine_learned", 2, True, True))
+# (name, storage, method, learned): method "sor" = red-black SOR on x (FP32),
+# "refine" = FP32 iterative refinement around inner SOR sweeps in `storage`,
+# "mg" = FP32 iterative refinement around one multigrid V-cycle per check.
+SOLVE_VARIANTS = (("f32_plain", 0, "sor", False), ("f32_learned", 0, "sor", True), ("f16_refine", 1, "refine", False),
+                  ("bf16_refine", 2, "refine", False), ("f16_refine_learned", 1, "refine", True),
+                  ("bf16_refine_learned", 2, "refine", True), ("f32_mg", 0, "mg", False), ("f16_mg", 1, "mg", False),
+                  ("bf16_mg", 2, "mg", False))
+
+
+def solve_args(method):
+    """PoissonBand.solve arguments of a method: SOR omega 1.9 with an FP64
+    residual check every 4 sweeps; multigrid V(2, 2)-cycles (Gauss-Seidel
+    smoothing) with a check after every cycle."""
+    if method == "mg":
+        return dict(multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 100)
+    return dict(refine=method == "refine", omega=1.9, sweeps_per_check=4, max_sweeps=20000)
 
 
 def solve_leg(pkg, lib, band, phi, b, reps, keep_x=False):
-    """Time to ||b - A x|| / ||b|| <= 1e-6 from x = 0 (red-black SOR, omega 1.9,
-    FP64 residual check every 4 sweeps) for each variant; best of `reps`
-    (msbg_poisson_stats.ms: HIP events around the whole solve)."""
+    """Time to ||b - A x|| / ||b|| <= 1e-6 from x = 0 for each variant; best of
+    `reps` (msbg_poisson_stats.ms: HIP events around the whole solve)."""
     from msbg_rs_amd.learned import WEIGHTS
     net = pkg.CorrectorNet.load(WEIGHTS, lib=lib)
     out, xs = {}, {}
-    for name, storage, refine, learned in SOLVE_VARIANTS:
+    for name, storage, method, learned in SOLVE_VARIANTS:
         best = None
         for _ in range(reps):
             x = band.field(0)
-            st = band.solve(x, b, storage=storage, refine=refine, omega=1.9, sweeps_per_check=4, max_sweeps=20000,
-                            tol=1e-6, learned=net if learned else None, phi=phi if learned else None)
+            st = band.solve(x, b, storage=storage, tol=1e-6, learned=net if learned else None,
+                            phi=phi if learned else None, **solve_args(method))
             if best is None or st["ms"] < best["ms"]:
                 best, xbest = st, x
         out[name] = {k: best[k] for k in ("ms", "sweeps", "checks", "corrections", "accepted", "rel_residual",
-                                          "converged")}
+                                          "converged", "levels")}
         if keep_x:
             xs[name] = xbest.get()
     base = out["f32_plain"]["ms"]

@@ -349,7 +349,9 @@ typedef struct {
                                 red-black smoothing (`storage` on the band, FP32 below), trilinear
                                 prolongation, FP64 conjugate gradients on the coarsest level
                                 (<= 1024 cells); sweeps_per_check = smoothing sweeps before and
-                                after the coarse correction on every level */
+                                after the coarse correction on every level; the solve also stops
+                                (converged = 0) after two cycles in a row that each take off less
+                                than 10% of the residual (x at its FP32 rounding floor) */
     float omega;          /* SOR factor, 0 < omega < 2 */
     int sweeps_per_check; /* red-black sweeps between FP64 residual checks (inner sweeps per refinement step) */
     int max_sweeps;       /* cap on the band's red-black sweeps (refine = 2: 2 sweeps_per_check per cycle) */

@@ -213,8 +213,12 @@ __global__ __launch_bounds__(256) void mac_splat_kernel(GridView vg, GridView mg
                     const float df2 = dist_sq + d_a + 0.25f;
                     float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
                     w = w * w * w;
-                    atomicAdd(&sm[row + ix - x0s], w * scale_mass);
-                    atomicAdd(&sv[row + ix - x0s], w * scale_mom);
+                    // Cells outside the kernel's sphere (about half of the
+                    // box) add +-0: skipped, the sums (from +0) keep their bits.
+                    if (w > 0.0f) {
+                        atomicAdd(&sm[row + ix - x0s], w * scale_mass);
+                        atomicAdd(&sv[row + ix - x0s], w * scale_mom);
+                    }
                 }
             }
         }

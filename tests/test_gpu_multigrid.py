@@ -138,3 +138,15 @@ def test_flip_projection_by_multigrid(pkg, hip):
     assert st["solve"]["sweeps"] <= 4 * 15, st
     n0, n1 = fs.divergence_norms()
     assert n1 <= 1e-4 * n0, (n0, n1)
+
+
+def test_multigrid_stops_at_the_fp32_floor(pkg, hip):
+    """An unreachable tolerance: the solve stops, unconverged, once two
+    V-cycles in a row take off less than 10% of the residual, with the
+    residual at the FP32 floor of this pressure (~2.3e-7 ||p|| / ||b||)."""
+    from test_gpu_flip import run_step
+    fs, ps, out, st = run_step(pkg, hip, 64, 200000, storage=1, refine=True, sweeps_per_check=2, max_sweeps=4 * 200,
+                               tol=1e-10, omega=1.0, multigrid=True)
+    sv = st["solve"]
+    assert not sv["converged"] and sv["sweeps"] <= 4 * 40, sv
+    assert sv["rel_residual"] <= 2e-6, sv
