@@ -167,31 +167,109 @@ __global__ void gather_payload_kernel(const float *__restrict__ pos, const float
     }
 }
 
-// One (direction a, colour) pass: workgroup k stages the particles of
-// occupied block particle_blocks[list[k]] and commits its window.
-__global__ __launch_bounds__(256) void mac_splat_kernel(GridView vg, GridView mg, int a, const int32_t *__restrict__ list,
-                                                        const int32_t *__restrict__ pblocks,
-                                                        const int32_t *__restrict__ starts,
-                                                        const float *__restrict__ spos, const float *__restrict__ svel,
-                                                        const float *__restrict__ smass, float r_p, float inv_r2, int m) {
-    extern __shared__ float smem[];
-    const int MSX = 16 + 2 * m, NS = MSX * MSX * MSX;
-    float *sm = smem, *sv = smem + NS;
-    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
-        sm[s] = 0.0f;
-        sv[s] = 0.0f;
-    }
+// A work item of the MAC splat: particles p0 .. p1 - 1 (sorted) of occupied
+// block k; `shared` when the block's particles are split over several items.
+struct SplatItem {
+    int32_t k, p0, p1, shared;
+};
+constexpr int kSplatThreads = 512;
+constexpr int kSplatChunk = 2048;  // particles per item (4 per lane)
+
+// Sum over the work-group (every thread receives it); s_red holds one slot per wave.
+__device__ __forceinline__ float splat_wg_sum(float v, float *s_red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
-    const int k = list[blockIdx.x];
-    const int bid = pblocks[k];
-    const int bx = bid % vg.nx, by = (bid / vg.nx) % vg.ny, bz = bid / (vg.nx * vg.ny);
+    float t = 0.0f;
+    for (int w = 0; w < kSplatThreads / 64; ++w) t += s_red[w];
+    __syncthreads();
+    return t;
+}
+
+// One (direction a, colour) pass of the MAC splat (p2g.rs:261-333):
+// workgroup i stages the particles of items[i] into its window and commits
+// it (with global atomics when other items of the same colour pass share the
+// block).  CH 0 splats the mass m w, CH 1 the momentum m u_a w, CH 2 both.
+//
+// The window accumulates in fixed point: with B >= the item's sum of
+// |contribution| and B < 2^e, every contribution c is scaled by 2^S,
+// S = 61 - e, so |window sum| 2^S < 2^61.  A scaled contribution of
+// magnitude >= 2^24 is an integer in float, and its int64 LDS add is exact:
+// those sums carry no rounding and do not depend on order.  The rare smaller
+// ones (below ~2^-37 B: a particle at the edge of a cell's support) are added
+// in float: into a float window (CH 0 / 1), or (CH 2, where two int64
+// windows leave no room for it) into a list of kResCap entries applied at the
+// commit; past kResCap they are rounded to the fixed-point grid (absolute
+// error <= 2^-S-1 each).  The commit adds (double) acc 2^-S + the float part,
+// rounded once.  Integer LDS atomics run ~3x faster than float ones here.
+constexpr int kResCap = 2048;
+constexpr int kSplatMaxCells = 16 * kSplatThreads;  // window cells of CH 2 (m <= 2: 20^3)
+
+template <int CH>
+__global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, GridView gv, int a,
+                                                                  const SplatItem *__restrict__ items,
+                                                                  const int32_t *__restrict__ pblocks,
+                                                                  const float *__restrict__ spos,
+                                                                  const float *__restrict__ svel,
+                                                                  const float *__restrict__ smass, float r_p,
+                                                                  float inv_r2, int m) {
+    extern __shared__ __attribute__((aligned(8))) unsigned char smem_raw[];
+    __shared__ float s_red[kSplatThreads / 64];
+    __shared__ int s_nres;
+    constexpr int NC = CH == 2 ? 2 : 1;  // channels of this pass
+    const int MSX = 16 + 2 * m, NS = MSX * MSX * MSX;
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem_raw);  // NC windows
+    float *res = reinterpret_cast<float *>(acc + NC * NS);                         // CH 0 / 1: float window
+    uint2 *rlist = reinterpret_cast<uint2 *>(acc + NC * NS);                       // CH 2: (cell | channel << 31, value)
+    for (int s = threadIdx.x; s < NC * NS; s += blockDim.x) acc[s] = 0ull;
+    if (CH != 2)
+        for (int s = threadIdx.x; s < NS; s += blockDim.x) res[s] = 0.0f;
+    if (threadIdx.x == 0) s_nres = 0;
+    const SplatItem item = items[blockIdx.x];
+    const int bid = pblocks[item.k];
+    const int bx = bid % gm.nx, by = (bid / gm.nx) % gm.ny, bz = bid / (gm.nx * gm.ny);
     const int b0x = bx * 16, b0y = by * 16, b0z = bz * 16;
     const int x0s = b0x - m, y0s = b0y - m, z0s = b0z - m, whi = 16 + m - 1;
     const float ofx = a == 0 ? 1.0f : 0.5f, ofy = a == 1 ? 1.0f : 0.5f, ofz = a == 2 ? 1.0f : 0.5f;
-    for (int pi = starts[k] + threadIdx.x; pi < starts[k + 1]; pi += blockDim.x) {  // p2g.rs:261-333
+    // Lane t takes a contiguous run of the item's particles.
+    const int p_beg = item.p0, p_cnt = item.p1 - item.p0;
+    const int per = (p_cnt + (int)blockDim.x - 1) / (int)blockDim.x;
+    // channel k of this pass is the mass when CH is 0, or CH is 2 and k is 0
+    auto is_mass = [](int k) { return CH == 2 ? k == 0 : CH == 0; };
+    auto scale_of = [&](int pi, int k) { return is_mass(k) ? smass[pi] : smass[pi] * svel[3 * pi + a]; };
+    float bsum[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) bsum[k] = 0.0f;
+    for (int j = 0; j < per; ++j) {
+        const int pi = p_beg + (int)threadIdx.x * per + j;
+        if (pi < p_beg + p_cnt)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) bsum[k] += fabsf(scale_of(pi, k));
+    }
+    // w <= 1; the FP32 sum of the bounds is within a factor 1 + 2^-12 of its
+    // value, and e leaves a factor 2 spare.
+    float up[NC];
+    int ex[NC];
+    bool any = false, finite = true;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const float bound = splat_wg_sum(bsum[k], s_red);
+        finite = finite && isfinite(bound);
+        any = any || bound > 0.0f;
+        frexpf(bound > 0.0f && isfinite(bound) ? bound : 1.0f, &ex[k]);  // bound < 2^e
+        up[k] = ldexpf(1.0f, 61 - ex[k]);                                 // 2^S
+    }
+    // Nothing but zero contributions, or non-finite particle data (whose sums
+    // have no fixed-point scale): the item adds nothing.  Uniform per group.
+    if (!any || !finite) return;
+    constexpr float kExact = 16777216.0f;  // 2^24
+    for (int j = 0; j < per; ++j) {
+        const int pi = p_beg + (int)threadIdx.x * per + j;
+        if (pi >= p_beg + p_cnt) break;
         const float px = spos[3 * pi], py = spos[3 * pi + 1], pz = spos[3 * pi + 2];
-        const float scale_mass = smass[pi];
-        const float scale_mom = smass[pi] * svel[3 * pi + a];
+        float sc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) sc[k] = scale_of(pi, k);
         const float px0 = px - 0.5f, py0 = py - 0.5f, pz0 = pz - 0.5f;
         const int ix1 = min(max((int)ceilf(px - r_p - ofx), b0x - m), b0x + whi);
         const int ix2 = min(max((int)floorf(px + r_p - ofx), b0x - m), b0x + whi);
@@ -213,21 +291,79 @@ __global__ __launch_bounds__(256) void mac_splat_kernel(GridView vg, GridView mg
                     const float df2 = dist_sq + d_a + 0.25f;
                     float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
                     w = w * w * w;
-                    // Cells outside the kernel's sphere (about half of the
-                    // box) add +-0: skipped, the sums (from +0) keep their bits.
-                    if (w > 0.0f) {
-                        atomicAdd(&sm[row + ix - x0s], w * scale_mass);
-                        atomicAdd(&sv[row + ix - x0s], w * scale_mom);
+                    const int cell = row + ix - x0s;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        const float c = w * sc[k];
+                        if (c == 0.0f) continue;  // adds +-0: the window keeps its value
+                        const float q = c * up[k];  // exact: a power-of-two scale within range
+                        if (fabsf(q) >= kExact) {
+                            atomicAdd(&acc[k * NS + cell], (unsigned long long)(long long)q);
+                        } else if (CH != 2) {
+                            atomicAdd(&res[cell], c);
+                        } else {
+                            const int slot = atomicAdd(&s_nres, 1);
+                            if (slot < kResCap)
+                                rlist[slot] = make_uint2((uint32_t)cell | (uint32_t)k << 31, __float_as_uint(c));
+                            else
+                                atomicAdd(&acc[k * NS + cell], (unsigned long long)llrintf(q));
+                        }
                     }
                 }
             }
         }
     }
     __syncthreads();
+    // Window values in float: CH 0 / 1 into the residue window; CH 2 into
+    // the first half of each int64 window's bytes (read all, barrier, write),
+    // then the listed small terms on top.
+    float *fv[NC];
+    if (CH != 2) {
+        const double down = ldexp(1.0, ex[0] - 61);
+        for (int s = threadIdx.x; s < NS; s += blockDim.x) {
+            const long long ia = (long long)acc[s];
+            res[s] = (float)((double)ia * down + (double)res[s]);
+        }
+        fv[0] = res;
+    } else {
+        float v[NC][kSplatMaxCells / kSplatThreads];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const double down = ldexp(1.0, ex[k] - 61);
+#pragma unroll
+            for (int i = 0; i < kSplatMaxCells / kSplatThreads; ++i) {
+                const int s = threadIdx.x + i * kSplatThreads;
+                v[k][i] = s < NS ? (float)((double)(long long)acc[k * NS + s] * down) : 0.0f;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            fv[k] = reinterpret_cast<float *>(acc + k * NS);
+#pragma unroll
+            for (int i = 0; i < kSplatMaxCells / kSplatThreads; ++i) {
+                const int s = threadIdx.x + i * kSplatThreads;
+                if (s < NS) fv[k][s] = v[k][i];
+            }
+        }
+        __syncthreads();
+        const int nl = min(s_nres, kResCap);
+        for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+            const uint2 e = rlist[i];
+            atomicAdd(&fv[e.x >> 31][e.x & 0x7fffffffu], __uint_as_float(e.y));
+        }
+    }
+    __syncthreads();
     // Commit (p2g.rs:342-411): staging cell -> its region -> the region's block.
     for (int s = threadIdx.x; s < NS; s += blockDim.x) {
-        const float vm = sm[s], vv = sv[s];
-        if (vm == 0.0f && vv == 0.0f) continue;
+        float val[NC];
+        bool nz = false;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            val[k] = fv[k][s];
+            nz = nz || val[k] != 0.0f;
+        }
+        if (!nz) continue;
         const int lx = s % MSX, ly = (s / MSX) % MSX, lz = s / (MSX * MSX);
         const int dx = lx < m ? -1 : (lx < m + 16 ? 0 : 1);
         const int dy = ly < m ? -1 : (ly < m + 16 ? 0 : 1);
@@ -237,14 +373,21 @@ __global__ __launch_bounds__(256) void mac_splat_kernel(GridView vg, GridView mg
         const int zlo = dz < 0 ? 0 : (dz == 0 ? m : m + 16), zlen = dz == 0 ? 16 : m;
         const int gx0 = b0x + xlo - m, gy0 = b0y + ylo - m, gz0 = b0z + zlo - m;
         if (gx0 < 0 || gy0 < 0 || gz0 < 0) continue;
-        if (gx0 + xlen > vg.sx || gy0 + ylen > vg.sy || gz0 + zlen > vg.sz) continue;
-        const int64_t nb = (int64_t)(bx + dx) + (int64_t)(by + dy) * vg.nx + (int64_t)(bz + dz) * vg.nx * vg.ny;
+        if (gx0 + xlen > gm.sx || gy0 + ylen > gm.sy || gz0 + zlen > gm.sz) continue;
+        const int64_t nb = (int64_t)(bx + dx) + (int64_t)(by + dy) * gm.nx + (int64_t)(bz + dz) * gm.nx * gm.ny;
         const int gx = x0s + lx, gy = y0s + ly, gz = z0s + lz;
         const int vid = (gx & 15) | ((gy & 15) << 4) | ((gz & 15) << 8);
-        const int ms = mg.b2s[nb], vs = vg.b2s[nb];
+        const int ms = gm.b2s[nb], vs = gv.b2s[nb];
         if (ms < 2 || vs < 2) continue;
-        if (vm != 0.0f) mg.slab[(int64_t)ms * 4096 + vid] += vm;
-        if (vv != 0.0f) vg.slab[(int64_t)vs * 4096 + vid] += vv;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            if (val[k] == 0.0f) continue;
+            float *po = is_mass(k) ? gm.slab + (int64_t)ms * 4096 + vid : gv.slab + (int64_t)vs * 4096 + vid;
+            if (item.shared)  // blocks of one colour never share a window: only the block's own items race
+                unsafeAtomicAdd(po, val[k]);
+            else
+                *po += val[k];
+        }
     }
 }
 
@@ -548,43 +691,68 @@ int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, co
     int64_t nocc = 0;
     rc = run_lengths(keys, n, pblocks, starts, &nocc);
     if (rc) return rc;
-    // colour lists of occupied-block indices (p2g.rs:186-191)
-    std::vector<int32_t> hb((size_t)nocc);
+    // colour lists of occupied-block indices (p2g.rs:186-191), each block's
+    // particles cut into items of at most kSplatChunk (dense blocks, e.g. a
+    // droplet's, would otherwise serialise a whole colour pass on one work-group)
+    std::vector<int32_t> hb((size_t)nocc), hs((size_t)nocc + 1);
     MSBG_HIP(hipMemcpy(hb.data(), pblocks, sizeof(int32_t) * nocc, hipMemcpyDeviceToHost));
-    std::vector<int32_t> order;
+    MSBG_HIP(hipMemcpy(hs.data(), starts, sizeof(int32_t) * (nocc + 1), hipMemcpyDeviceToHost));
+    std::vector<SplatItem> order;
     int64_t cnt[8] = {0};
     for (int c = 0; c < 8; ++c)
         for (int64_t i = 0; i < nocc; ++i) {
             const int64_t b = hb[i];
             const int64_t bx = b % g->nx, by = (b / g->nx) % g->ny, bz = b / g->nxy;
-            if ((int)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) == c) {
-                order.push_back((int32_t)i);
+            if ((int)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) != c) continue;
+            const int32_t p0 = hs[i], p1 = hs[i + 1];
+            const int32_t shared = p1 - p0 > kSplatChunk ? 1 : 0;
+            for (int32_t q = p0; q < p1; q += kSplatChunk) {
+                order.push_back(SplatItem{(int32_t)i, q, std::min(p1, q + kSplatChunk), shared});
                 ++cnt[c];
             }
         }
-    int32_t *d_order = (int32_t *)workspace(3, sizeof(int32_t) * std::max<int64_t>(nocc, 1), &rc);
+    SplatItem *d_order =
+        (SplatItem *)workspace(3, sizeof(SplatItem) * std::max<size_t>(order.size(), 1), &rc);
     if (rc) return rc;
-    MSBG_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * nocc, hipMemcpyHostToDevice, stream()));
+    MSBG_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(SplatItem) * order.size(), hipMemcpyHostToDevice, stream()));
     const int m = (int)ceilf(r_p);
     const int msx = 16 + 2 * m;
-    const size_t lds = sizeof(float) * 2 * msx * msx * msx;
+    // both channels in one pass when two int64 windows and the residue list
+    // fit (r_p <= 2), else a pass per channel with a float residue window
+    const bool both = msx * msx * msx <= kSplatMaxCells;
+    const size_t lds = both ? 2 * sizeof(unsigned long long) * msx * msx * msx + sizeof(uint2) * kResCap
+                            : (sizeof(unsigned long long) + sizeof(float)) * msx * msx * msx;
     {
         int dev = 0, max_lds = 0;
         MSBG_HIP(hipGetDevice(&dev));
         MSBG_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         MSBG_CHECK((int64_t)lds <= max_lds, MSBG_EINVAL,
                    "particles_to_grid: the staging window for this r_p exceeds the workgroup LDS limit");
-        if (lds > 65536)
-            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)lds));
+        if (lds > 65536) {
+            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<0>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<1>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<2>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
     }
     const float inv_r2 = 1.0f / (r_p * r_p);
     for (int a = 0; a < 3; ++a) {
         int64_t off = 0;
         for (int c = 0; c < 8; ++c) {
-            if (cnt[c])
-                hipLaunchKernelGGL(mac_splat_kernel, dim3((unsigned)cnt[c]), dim3(256), lds, stream(), vel[a]->view(),
-                                   mass[a]->view(), a, d_order + off, pblocks, starts, spos, svel, smass, r_p, inv_r2, m);
+            if (cnt[c] && both) {
+                hipLaunchKernelGGL(mac_splat_kernel<2>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
+                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
+                                   inv_r2, m);
+            } else if (cnt[c]) {
+                hipLaunchKernelGGL(mac_splat_kernel<0>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
+                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
+                                   inv_r2, m);
+                hipLaunchKernelGGL(mac_splat_kernel<1>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
+                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
+                                   inv_r2, m);
+            }
             off += cnt[c];
         }
         MSBG_HIP(hipGetLastError());
