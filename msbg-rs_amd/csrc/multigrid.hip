@@ -176,25 +176,30 @@ __global__ __launch_bounds__(kT) void init_blk_kernel(i64 n_active, const PBlock
 
 // b_c = 0.5 * sum of the residuals of its children (structured level of B
 // cells per edge into the level of B / 2; B = 2 gives one cell per block).
+// Lane 8 j + o evaluates child o of coarse cell j; the 8 lanes of a coarse
+// cell sum by shuffles in a fixed order.
 template <int B>
 __global__ __launch_bounds__(kT) void restrict_blk_kernel(i64 n_active, const PBlock *__restrict__ blocks, Dom d,
                                                           const int32_t *__restrict__ nb27,
                                                           const float *__restrict__ x, const float *__restrict__ b,
                                                           float w, float *bc) {
     constexpr int C = B / 2, C3 = C * C * C;
-    const i64 i = (i64)blockIdx.x * kT + threadIdx.x;
-    if (i >= n_active * C3) return;
-    const i64 k = i / C3;
-    const int v = (int)(i % C3), ux = v % C, uy = (v / C) % C, uz = v / (C * C);
-    const PBlock Bk = blocks[k];
-    const int32_t *T = nb27 + k * 27;
-    float sum = 0.0f, dg;
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
+    const i64 t = (i64)blockIdx.x * kT + threadIdx.x;
+    const i64 i = t >> 3;  // coarse cell
+    const int o = (int)(t & 7);
+    float r = 0.0f;
+    if (i < n_active * C3) {
+        const i64 k = i / C3;
+        const int v = (int)(i % C3), ux = v % C, uy = (v / C) % C, uz = v / (C * C);
+        const PBlock Bk = blocks[k];
         const int fx = 2 * ux + (o & 1), fy = 2 * uy + ((o >> 1) & 1), fz = 2 * uz + (o >> 2);
-        if (live(Bk, d, 16 / B, fx, fy, fz)) sum += blk_residual<B>(x, b, T, Bk, d, k, fx, fy, fz, w, &dg);
+        float dg;
+        if (live(Bk, d, 16 / B, fx, fy, fz)) r = blk_residual<B>(x, b, nb27 + k * 27, Bk, d, k, fx, fy, fz, w, &dg);
     }
-    bc[i] = 0.5f * sum;
+    r += __shfl_xor(r, 1);
+    r += __shfl_xor(r, 2);
+    r += __shfl_xor(r, 4);
+    if (o == 0 && i < n_active * C3) bc[i] = 0.5f * r;
 }
 
 // Trilinear value at fine cell (cx, cy, cz) of listed block k from the
@@ -421,19 +426,20 @@ __global__ __launch_bounds__(kT) void prolong_list_kernel(i64 n, const int32_t *
     x[i] += v;
 }
 
-// Sum over the work-group in a fixed order (lane butterfly, then waves 0..15
+// Sum over the work-group in a fixed order (lane butterfly, then the waves
 // in order); every thread receives the same value.
 __device__ __forceinline__ double wg_sum(double v, double *s_red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0.0;
-    for (int w = 0; w < kCoarseMax / 64; ++w) t += s_red[w];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_red[w];
     __syncthreads();
     return t;
 }
 
-// Coarsest level: conjugate gradients in FP64, one cell per thread, until
+// Coarsest level: conjugate gradients in FP64, one cell per thread (the
+// group holds the cells rounded up to whole waves, <= kCoarseMax), until
 // ||r|| <= kCgTol ||b|| or kCgMaxIter iterations (the loop condition is the
 // same on every thread, so every barrier is reached by the whole group).
 __global__ __launch_bounds__(kCoarseMax) void coarse_cg_kernel(int n, const int32_t *__restrict__ nb,
@@ -520,12 +526,13 @@ void smooth(const msbg_poisson *p, const Level &L, hipStream_t s, float omega, i
 void restrict_to(const msbg_poisson *p, const Level &F, const Level &Cl, hipStream_t s) {
     const i64 na = p->n_active;
     const int32_t *T = p->mg->nb27;
+    const int g8 = grid_of(8 * Cl.n);  // a lane per child
     if (F.B == 8)
-        restrict_blk_kernel<8><<<grid_of(Cl.n), kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
+        restrict_blk_kernel<8><<<g8, kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
     else if (F.B == 4)
-        restrict_blk_kernel<4><<<grid_of(Cl.n), kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
+        restrict_blk_kernel<4><<<g8, kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
     else if (F.B == 2)
-        restrict_blk_kernel<2><<<grid_of(Cl.n), kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
+        restrict_blk_kernel<2><<<g8, kT, 0, s>>>(na, p->d_blocks, p->dom, T, F.x, F.b, F.w, Cl.b);
     else
         restrict_list_kernel<<<grid_of(Cl.n), kT, 0, s>>>(Cl.n, Cl.ch, F.nb, F.cf, F.diag, F.x, F.b, Cl.b);
 }
@@ -799,7 +806,8 @@ void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu) {
         restrict_to(p, L, H.lv[k + 1], s);
     }
     const Level &Lc = H.lv[nl - 1];
-    coarse_cg_kernel<<<1, kCoarseMax, 0, s>>>((int)Lc.n, Lc.nb, Lc.cf, Lc.diag, Lc.b, Lc.x);
+    const int cg_threads = (int)std::min<i64>(kCoarseMax, std::max<i64>(64, (Lc.n + 63) / 64 * 64));
+    coarse_cg_kernel<<<1, cg_threads, 0, s>>>((int)Lc.n, Lc.nb, Lc.cf, Lc.diag, Lc.b, Lc.x);
     for (int k = nl - 2; k >= 0; --k) {
         const Level &L = H.lv[k];
         prolong_to(p, H.lv[k + 1], L, s);
