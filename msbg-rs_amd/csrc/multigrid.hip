@@ -621,11 +621,12 @@ void level3_op(const msbg_poisson *p, HostOp &H) {
     }
 }
 
-// The next coarser level of F (Galerkin / 4), its child table and F's
-// interpolation table.  Coarse cells are numbered in the order of their
+// The next coarser level of F (Galerkin / 4) and, with `tables`, its child
+// table and F's interpolation table (level 4 needs neither: its transfers are
+// the structured kernels).  Coarse cells are numbered in the order of their
 // first child, so level 4's cell k is listed block k.
 void aggregate(const msbg_poisson *p, const HostOp &F, HostOp &Cl, std::vector<int32_t> &ch,
-               std::vector<int32_t> &pn) {
+               std::vector<int32_t> &pn, bool tables = true) {
     Cl.l = F.l + 1;
     for (int a = 0; a < 3; ++a) Cl.dims[a] = (F.dims[a] + 1) >> 1;
     Cl.dense.assign((size_t)(Cl.dims[0] * Cl.dims[1] * Cl.dims[2]), -1);
@@ -658,6 +659,9 @@ void aggregate(const msbg_poisson *p, const HostOp &F, HostOp &Cl, std::vector<i
             Cl.cf[(size_t)c * 6 + f] += 0.25f * F.cf[i * 6 + f];
         }
     }
+    ch.clear();
+    pn.clear();
+    if (!tables) return;
     ch.assign(nc * 8, -1);
     for (size_t c = 0; c < nc; ++c)
         for (int o = 0; o < 8; ++o) {
@@ -765,7 +769,7 @@ int ensure(msbg_poisson *p) {
     HostOp fine, cur;
     level3_op(p, fine);
     std::vector<int32_t> ch, pn;
-    aggregate(p, fine, cur, ch, pn);  // level 4; its children are level 3's structured cells
+    aggregate(p, fine, cur, ch, pn, false);  // level 4; its children are level 3's structured cells
     bool first = true;
     while (e == hipSuccess) {
         Level L;

@@ -580,6 +580,9 @@ void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, con
     }
 }
 
+// Bands from this many cells up run their V-cycles as direct launches.
+constexpr int64_t kDirectCycleCells = (int64_t)64 << 20;
+
 int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const float *b, float omega, int sweeps) {
     if (!(p->exec && p->g_mode == mode && p->g_storage == storage && p->g_sweeps == sweeps && p->g_omega == omega &&
           p->g_x == x && p->g_b == b)) {
@@ -914,7 +917,16 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
         // fine-level red-black sweeps of one check period
         const int per = mode == 2 ? 2 * prm->sweeps_per_check : prm->sweeps_per_check;
         if (!std::isfinite(rel) || st->sweeps + per > prm->max_sweeps) break;
-        rc = run_period_graph(p, mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+        if (mode == 2 && p->n_cells >= kDirectCycleCells) {
+            // a cycle of a band this large is milliseconds of GPU work, which
+            // hides its ~60 launches; a graph would cost its instantiation
+            // (~4 ms) on every new band
+            period(p, stream(), mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess) rc = hip_fail(le, "msbg_poisson_solve");
+        } else {
+            rc = run_period_graph(p, mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+        }
         if (rc) break;
         st->sweeps += per;
         st->checks += 1;
