@@ -172,7 +172,7 @@ __global__ void gather_payload_kernel(const float *__restrict__ pos, const float
 struct SplatItem {
     int32_t k, p0, p1, shared;
 };
-constexpr int kSplatThreads = 512;
+constexpr int kSplatThreads = 1024;  // one group per CU holds the 144 KB window of r_p <= 2: 16 waves
 constexpr int kSplatChunk = 2048;  // particles per item (4 per lane)
 
 // Sum over the work-group (every thread receives it); s_red holds one slot per wave.
@@ -203,7 +203,7 @@ __device__ __forceinline__ float splat_wg_sum(float v, float *s_red) {
 // error <= 2^-S-1 each).  The commit adds (double) acc 2^-S + the float part,
 // rounded once.  Integer LDS atomics run ~3x faster than float ones here.
 constexpr int kResCap = 2048;
-constexpr int kSplatMaxCells = 16 * kSplatThreads;  // window cells of CH 2 (m <= 2: 20^3)
+constexpr int kSplatMaxCells = 8192;  // window cells of CH 2 (m <= 2: 20^3)
 
 template <int CH>
 __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, GridView gv, int a,
