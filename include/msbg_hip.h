@@ -52,6 +52,10 @@ int msbg_free(void *dev_ptr);
 int msbg_memcpy_htod(void *dev_dst, const void *host_src, size_t bytes);
 int msbg_memcpy_dtoh(void *host_dst, const void *dev_src, size_t bytes);
 int msbg_memcpy_dtod(void *dev_dst, const void *dev_src, size_t bytes);
+/* Fills n_words 32-bit words of device memory with `value` (a kernel on the
+ * library stream, asynchronous; e.g. a zero field without a host upload).  No
+ * reference counterpart: device-memory utility of the C ABI. */
+int msbg_fill_u32(void *dev_ptr, int64_t n_words, uint32_t value);
 /* Timing with HIP events on the library stream. */
 int msbg_event_create(void **ev);
 int msbg_event_destroy(void *ev);

@@ -118,6 +118,12 @@ int msbg_memcpy_dtod(void *d, const void *s, size_t bytes) {
     return MSBG_OK;
 }
 
+int msbg_fill_u32(void *d, int64_t n_words, uint32_t value) {
+    MSBG_CHECK(n_words >= 0 && (n_words == 0 || d), MSBG_EINVAL, "msbg_fill_u32: bad buffer");
+    MSBG_CHECK(((uintptr_t)d & 3u) == 0, MSBG_EINVAL, "msbg_fill_u32: the buffer must be 4-byte aligned");
+    return fill_f32(reinterpret_cast<float *>(d), n_words, __builtin_bit_cast(float, value));
+}
+
 int msbg_event_create(void **ev) {
     MSBG_CHECK(ev, MSBG_EINVAL, "msbg_event_create: null out");
     hipEvent_t e;

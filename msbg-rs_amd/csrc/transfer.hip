@@ -173,7 +173,7 @@ struct SplatItem {
     int32_t k, p0, p1, shared;
 };
 constexpr int kSplatThreads = 1024;  // one group per CU holds the 144 KB window of r_p <= 2: 16 waves
-constexpr int kSplatChunk = 2048;  // particles per item (4 per lane)
+constexpr int kSplatChunk = 4096;  // particles per item (4 per lane)
 
 // Sum over the work-group (every thread receives it); s_red holds one slot per wave.
 __device__ __forceinline__ float splat_wg_sum(float v, float *s_red) {

@@ -26,6 +26,7 @@ _SIGS = {
     "msbg_memcpy_htod": [vp, vp, C.c_size_t],
     "msbg_memcpy_dtoh": [vp, vp, C.c_size_t],
     "msbg_memcpy_dtod": [vp, vp, C.c_size_t],
+    "msbg_fill_u32": [vp, i64, C.c_uint32],
     "msbg_event_create": [PP],
     "msbg_event_destroy": [vp],
     "msbg_event_record": [vp],

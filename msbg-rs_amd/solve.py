@@ -60,10 +60,13 @@ class PoissonBand:
         self.n_active, self.n_cells, self.n_slots = a.value, c.value, s.value
 
     def field(self, storage=Storage.F32, host=None):
-        """A device field (DeviceArray of n_slots * 4096 elements), zero or from `host`."""
+        """A device field (DeviceArray of n_slots * 4096 elements), zero (filled
+        on the device) or from `host`."""
         n = self.n_slots * 4096
         if host is None:
-            host = np.zeros(n, STORAGE_DTYPE[storage])
+            f = self.lib.empty(n, STORAGE_DTYPE[storage])
+            self.lib.call("msbg_fill_u32", f.ptr, f.nbytes // 4, 0)
+            return f
         host = np.ascontiguousarray(host).reshape(-1)
         assert host.size == n and host.dtype == np.dtype(STORAGE_DTYPE[storage])
         return self.lib.to_device(host)

@@ -56,3 +56,19 @@ def test_grid_copy_and_block_slots(pkg, hip):
     from msbg_rs_amd.lib import MsbgError
     with pytest.raises(MsbgError):
         small.copy_from(g)
+
+
+def test_fill_u32(pkg, hip):
+    """msbg_fill_u32 writes exactly the words asked for; a misaligned pointer
+    is refused before any launch."""
+    from msbg_rs_amd.lib import MsbgError
+    a = hip.to_device(np.arange(1000, dtype=np.uint32))
+    hip.call("msbg_fill_u32", a.ptr + 4 * 10, 977, 0xDEADBEEF)
+    got = a.get()
+    want = np.arange(1000, dtype=np.uint32)
+    want[10:987] = 0xDEADBEEF
+    assert np.array_equal(got, want)
+    hip.call("msbg_fill_u32", a.ptr, 0, 7)  # nothing to do
+    with pytest.raises(MsbgError):
+        hip.call("msbg_fill_u32", a.ptr + 2, 4, 0)
+    assert np.array_equal(a.get(), want)
