@@ -65,6 +65,10 @@ struct Level {
     uint8_t *col = nullptr;   // (x + y + z) & 1 of the cell's lattice coordinates
     int32_t *ch = nullptr;    // [n][8]: children in the next finer cell list (level >= 5), -1 none
     int32_t *pn = nullptr;    // [n][8]: interpolation partners in the next coarser level, -1 none
+    // coarsest level: connected components, and which have no Dirichlet coupling
+    int16_t *comp = nullptr;  // [n]: component of each cell
+    uint8_t *sing = nullptr;  // [n_comp]: 1 when the component's operator is singular
+    int n_comp = 0, n_sing = 0;
 };
 
 struct Hierarchy {
@@ -438,15 +442,48 @@ __device__ __forceinline__ double wg_sum(double v, double *s_red) {
     return t;
 }
 
+// Removes, on each singular component (no Dirichlet coupling: the constants
+// are its null space), the component's mean from v.  Thread c < n_comp sums
+// component c in cell order (deterministic).
+__device__ void project_singular(double &v, bool in, int n, int n_comp, const int16_t *s_comp,
+                                 const uint8_t *__restrict__ sing, double *s_val, double *s_mean) {
+    const int t = threadIdx.x;
+    s_val[t] = in ? v : 0.0;
+    __syncthreads();
+    if (t < n_comp) {
+        double sum = 0.0;
+        int cnt = 0;
+        if (sing[t])
+            for (int i = 0; i < n; ++i)
+                if (s_comp[i] == t) {
+                    sum += s_val[i];
+                    ++cnt;
+                }
+        s_mean[t] = cnt ? sum / cnt : 0.0;
+    }
+    __syncthreads();
+    if (in && sing[s_comp[t]]) v -= s_mean[s_comp[t]];
+    __syncthreads();
+}
+
 // Coarsest level: conjugate gradients in FP64, one cell per thread (the
 // group holds the cells rounded up to whole waves, <= kCoarseMax), until
 // ||r|| <= kCgTol ||b|| or kCgMaxIter iterations (the loop condition is the
-// same on every thread, so every barrier is reached by the whole group).
+// same on every thread, so every barrier is reached by the whole group).  On
+// a component without Dirichlet coupling the operator is singular: b is
+// projected onto its range first and x onto the zero-mean space after, so
+// an inconsistent right-hand side (or rounding) cannot drive x along the
+// constants.
 __global__ __launch_bounds__(kCoarseMax) void coarse_cg_kernel(int n, const int32_t *__restrict__ nb,
                                                                const float *__restrict__ cf,
                                                                const float *__restrict__ diag,
-                                                               const float *__restrict__ b, float *x) {
+                                                               const float *__restrict__ b, float *x,
+                                                               const int16_t *__restrict__ comp,
+                                                               const uint8_t *__restrict__ sing, int n_comp,
+                                                               int n_sing) {
     __shared__ double s_p[kCoarseMax];
+    __shared__ double s_mean[kCoarseMax];
+    __shared__ int16_t s_comp[kCoarseMax];
     __shared__ double s_red[kCoarseMax / 64];
     const int t = threadIdx.x;
     const bool in = t < n;
@@ -461,6 +498,10 @@ __global__ __launch_bounds__(kCoarseMax) void coarse_cg_kernel(int n, const int3
         }
         dg = (double)diag[t];
         r = (double)b[t];
+    }
+    if (n_sing > 0) {
+        s_comp[t] = in ? comp[t] : (int16_t)0;
+        project_singular(r, in, n, n_comp, s_comp, sing, s_p, s_mean);
     }
     double xi = 0.0, p = r;
     double rr = wg_sum(r * r, s_red);
@@ -484,6 +525,7 @@ __global__ __launch_bounds__(kCoarseMax) void coarse_cg_kernel(int n, const int3
         p = r + (rr1 / rr) * p;
         rr = rr1;
     }
+    if (n_sing > 0) project_singular(xi, in, n, n_comp, s_comp, sing, s_p, s_mean);
     if (in) x[t] = (float)xi;
 }
 
@@ -689,6 +731,45 @@ void aggregate(const msbg_poisson *p, const HostOp &F, HostOp &Cl, std::vector<i
 }
 
 template <class T>
+hipError_t upload(T **d, const std::vector<T> &h);
+
+// Connected components of a level's link graph, and which of them have no
+// Dirichlet coefficient (singular operator), uploaded into L.
+hipError_t components(const HostOp &H, Level &L) {
+    const size_t n = H.n();
+    std::vector<int16_t> comp(n, -1);
+    std::vector<uint8_t> sing;
+    std::vector<int32_t> stack;
+    for (size_t s0 = 0; s0 < n; ++s0) {
+        if (comp[s0] >= 0) continue;
+        const int16_t id = (int16_t)sing.size();
+        double dir = 0.0;
+        stack.assign(1, (int32_t)s0);
+        comp[s0] = id;
+        while (!stack.empty()) {
+            const int32_t i = stack.back();
+            stack.pop_back();
+            dir += H.dir[i];
+            for (int f = 0; f < 6; ++f) {
+                const int32_t j = H.nb[(size_t)i * 6 + f];
+                if (j >= 0 && comp[j] < 0) {
+                    comp[j] = id;
+                    stack.push_back(j);
+                }
+            }
+        }
+        sing.push_back(dir > 0.0 ? 0 : 1);
+    }
+    L.n_comp = (int)sing.size();
+    L.n_sing = 0;
+    for (uint8_t v : sing) L.n_sing += v;
+    if (!L.n_sing) return hipSuccess;
+    hipError_t e = upload(&L.comp, comp);
+    if (e == hipSuccess) e = upload(&L.sing, sing);
+    return e;
+}
+
+template <class T>
 hipError_t upload(T **d, const std::vector<T> &h) {
     if (h.empty()) return hipSuccess;
     hipError_t e = hipMalloc(d, sizeof(T) * h.size());
@@ -729,6 +810,8 @@ void release(msbg_poisson *p) {
         hipFree(L.col);
         hipFree(L.ch);
         hipFree(L.pn);
+        hipFree(L.comp);
+        hipFree(L.sing);
     }
     hipFree(H->nb27);
     delete H;
@@ -778,6 +861,7 @@ int ensure(msbg_poisson *p) {
         e = upload_list(cur, L);
         if (e == hipSuccess && !first) e = upload(&L.ch, ch);
         const bool last = L.n <= kCoarseMax;
+        if (last && e == hipSuccess) e = components(cur, L);
         HostOp next;
         if (!last && e == hipSuccess) {
             aggregate(p, cur, next, ch, pn);
@@ -811,7 +895,8 @@ void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu) {
     }
     const Level &Lc = H.lv[nl - 1];
     const int cg_threads = (int)std::min<i64>(kCoarseMax, std::max<i64>(64, (Lc.n + 63) / 64 * 64));
-    coarse_cg_kernel<<<1, cg_threads, 0, s>>>((int)Lc.n, Lc.nb, Lc.cf, Lc.diag, Lc.b, Lc.x);
+    coarse_cg_kernel<<<1, cg_threads, 0, s>>>((int)Lc.n, Lc.nb, Lc.cf, Lc.diag, Lc.b, Lc.x, Lc.comp, Lc.sing,
+                                              Lc.n_comp, Lc.n_sing);
     for (int k = nl - 2; k >= 0; --k) {
         const Level &L = H.lv[k];
         prolong_to(p, H.lv[k + 1], L, s);

@@ -175,3 +175,43 @@ def test_large_band_runs_cycles_without_a_graph(pkg, hip):
     assert st["converged"] and st["sweeps"] <= 4 * 10 and st["levels"] >= 7, st
     r2, b2 = band.residual(0, x, b)
     assert np.sqrt(r2 / b2) == pytest.approx(st["rel_residual"], rel=1e-9)
+
+
+def test_single_block_band(pkg, orc, hip):
+    """One listed block in the middle of a 48^3 domain, every other block
+    absent (zero Dirichlet around it): levels 1-3 lie inside the block, level
+    4 is its single cell and the coarsest; the solve converges and the oracle
+    agrees."""
+    rng = np.random.default_rng(5)
+    g = pkg.SparseGrid("band", 48, 48, 48, 0.0, 0.0)
+    g.ensure_blocks_parallel(np.array([13]))
+    active = np.array([13], np.int64)
+    b = np.zeros((g.n_slots, 4096), np.float32)
+    b[g.b2s[13]] = rng.uniform(-1.0, 1.0, 4096).astype(np.float32)
+    _, pb = setup(pkg, hip, g, active)
+    dx, db = pb.field(0), hip.to_device(np.ascontiguousarray(b))
+    st = pb.solve(dx, db, storage=0, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 30, tol=1e-6)
+    assert st["converged"] and st["levels"] == 5 and st["sweeps"] <= 4 * 10, st
+    check_residual(orc, g, active, dx.get().reshape(-1, 4096), b, st, 1e-6)
+
+
+def test_closed_box_without_dirichlet_faces_ends_cleanly(pkg, hip):
+    """Every block of a 64^3 box listed: the band has no Dirichlet face and
+    its operator is singular (constants are in its null space).  A right-hand
+    side with a nonzero mean cannot be met; the solve must end (stall rule or
+    sweep cap) with finite values rather than run away.  With the mean
+    removed it is consistent and converges."""
+    g, active, x, b = band_problem(pkg, 64, 64, 64, 6, 1.0, n_full=0, n_unlisted=0)
+    assert active.size == 64
+    _, pb = setup(pkg, hip, g, active)
+    dx, db = pb.field(0), hip.to_device(np.ascontiguousarray(b))
+    st = pb.solve(dx, db, storage=0, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 40, tol=1e-6)
+    assert not st["converged"] and st["sweeps"] <= 4 * 40, st
+    assert np.all(np.isfinite(dx.get()))
+    bc = b.copy()
+    slots = g.b2s[active]
+    bc[slots] -= bc[slots].mean()
+    dx, db = pb.field(0), hip.to_device(np.ascontiguousarray(bc))
+    st = pb.solve(dx, db, storage=0, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 40, tol=1e-5)
+    assert st["converged"], st
+    assert np.all(np.isfinite(dx.get()))
