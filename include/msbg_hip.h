@@ -241,6 +241,12 @@ int msbg_mac_sample(msbg_grid *const *vel, const float *d_pos, int64_t n, int in
 /* g2p_blend -- fluid/g2p.rs:42-45 over n x 3 velocities. */
 int msbg_g2p_blend(const float *d_u_old, const float *d_u, const float *d_u_tilde, int64_t n, float alpha,
                    float *d_out);
+/* The G2P of a FLIP step -- fluid/g2p.rs:11-45 composed per particle:
+ * u = MacSampler(vel).sample(p), u* = MacSampler(vel_tilde).sample(p),
+ * out = g2p_blend(u_old, u, u*, alpha).  Same arithmetic (and bits) as two
+ * msbg_mac_sample calls and msbg_g2p_blend, in one pass over the particles. */
+int msbg_g2p_flip(msbg_grid *const *vel, msbg_grid *const *vel_tilde, const float *d_pos, const float *d_u_old,
+                  int64_t n, int interp, int bc, float alpha, float *d_out);
 /* particles_to_grid -- fluid/p2g.rs:141-164: bucket by block, active blocks
  * (footprint r_p + 1) ensured and zeroed on all six channels
  * (MacGrid::zero_blocks, mac.rs:95-102), the cubic-falloff splat of mass and

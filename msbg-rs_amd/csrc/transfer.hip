@@ -89,10 +89,11 @@ __device__ __forceinline__ void cubic_weights_dev(float t, float *w) {  // math/
     w[3] = (1.0f / 6.0f) * t * t * t;
 }
 
-// Sample::sample (math/sample.rs:494-526).  interp 0 Linear, 1 CubicBSpline;
+// Sample::sample (math/sample.rs:494-526).  IP 0 Linear, 1 CubicBSpline;
 // align 0 Corner, 1 CellCentered (cubic is always cell-centred).
-__device__ float sample_dev(const GridView &g, float px, float py, float pz, int interp, int align, int bc) {
-    if (interp == 0) {
+template <int IP>
+__device__ __forceinline__ float sample_ip(const GridView &g, float px, float py, float pz, int align, int bc) {
+    if constexpr (IP == 0) {
         float x = px, y = py, z = pz;
         if (align == 1) {
             x = x - 0.5f;
@@ -106,7 +107,7 @@ __device__ float sample_dev(const GridView &g, float px, float py, float pz, int
         const float c00 = lerp_f(c[0], c[1], u), c10 = lerp_f(c[2], c[3], u);
         const float c01 = lerp_f(c[4], c[5], u), c11 = lerp_f(c[6], c[7], u);
         return lerp_f(lerp_f(c00, c10, v), lerp_f(c01, c11, v), w);
-    }
+    } else {
     const float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
     const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
     float win[64];
@@ -119,16 +120,32 @@ __device__ float sample_dev(const GridView &g, float px, float py, float pz, int
     for (int k = 0; k < 4; ++k)
         for (int j = 0; j < 4; ++j) val = __builtin_fmaf(dot4_seq(win + 16 * k + 4 * j, wx), wy[j] * wz[k], val);
     return val;
+    }
+}
+
+__device__ float sample_dev(const GridView &g, float px, float py, float pz, int interp, int align, int bc) {
+    return interp == 0 ? sample_ip<0>(g, px, py, pz, align, bc) : sample_ip<1>(g, px, py, pz, align, bc);
 }
 
 // out[i * stride] = sample at p_i - 1/2 e_axis (axis < 0: no offset).
-__global__ void sample_kernel(GridView g, const float *__restrict__ pos, int64_t n, int interp, int align, int bc,
-                              int axis, float *__restrict__ out, int stride) {
+template <int IP>
+__global__ void sample_kernel(GridView g, const float *__restrict__ pos, int64_t n, int align, int bc, int axis,
+                              float *__restrict__ out, int stride) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-        if (axis >= 0) p[axis] = p[axis] - 0.5f;
-        out[i * stride] = sample_dev(g, p[0], p[1], p[2], interp, align, bc);
+        const float q[3] = {axis == 0 ? pos[3 * i] - 0.5f : pos[3 * i], axis == 1 ? pos[3 * i + 1] - 0.5f : pos[3 * i + 1],
+                            axis == 2 ? pos[3 * i + 2] - 0.5f : pos[3 * i + 2]};
+        out[i * stride] = sample_ip<IP>(g, q[0], q[1], q[2], align, bc);
     }
+}
+
+void launch_sample(const GridView &g, const float *pos, int64_t n, int interp, int align, int bc, int axis,
+                   float *out, int stride) {
+    if (interp == 0)
+        hipLaunchKernelGGL(sample_kernel<0>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), g, pos, n, align, bc, axis,
+                           out, stride);
+    else
+        hipLaunchKernelGGL(sample_kernel<1>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), g, pos, n, align, bc, axis,
+                           out, stride);
 }
 
 __global__ void blend_kernel(const float *__restrict__ u_old, const float *__restrict__ u,
@@ -136,6 +153,29 @@ __global__ void blend_kernel(const float *__restrict__ u_old, const float *__res
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
         const float delta = u[i] - u_tilde[i];  // g2p.rs:43-44
         out[i] = (u_old[i] + delta) * alpha + u[i] * (1.0f - alpha);
+    }
+}
+
+// G2P of a FLIP step, per particle (fluid/g2p.rs:11-45): the staggered
+// samples of u and u* (MacSampler::sample: face a at p - 1/2 e_a) and the
+// FLIP/PIC blend, with the arithmetic of sample_kernel and blend_kernel.
+struct MacViews {
+    GridView v[3];
+};
+template <int IP>
+__global__ void g2p_flip_kernel(MacViews u, MacViews ut, const float *__restrict__ pos,
+                                const float *__restrict__ u_old, int64_t n, int bc, float alpha,
+                                float *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+#pragma unroll 1
+        for (int a = 0; a < 3; ++a) {  // one axis at a time keeps the live registers of one sample pair
+            const float q[3] = {a == 0 ? p[0] - 0.5f : p[0], a == 1 ? p[1] - 0.5f : p[1], a == 2 ? p[2] - 0.5f : p[2]};
+            const float su = sample_ip<IP>(u.v[a], q[0], q[1], q[2], 1, bc);
+            const float st = sample_ip<IP>(ut.v[a], q[0], q[1], q[2], 1, bc);
+            const float delta = su - st;  // g2p.rs:43-44
+            out[3 * i + a] = (u_old[3 * i + a] + delta) * alpha + su * (1.0f - alpha);
+        }
     }
 }
 
@@ -616,8 +656,7 @@ int msbg_sample(const msbg_grid *g, const float *d_pos, int64_t n, int interp, i
     MSBG_CHECK((align == 0 || align == 1) && bc >= 0 && bc <= 2, MSBG_EINVAL, "sample: bad alignment / boundary");
     MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "sample: null argument");
     if (n == 0) return MSBG_OK;
-    hipLaunchKernelGGL(sample_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), g->view(), d_pos, n, interp, align,
-                       bc, -1, d_out, 1);
+    launch_sample(g->view(), d_pos, n, interp, align, bc, -1, d_out, 1);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;
@@ -632,8 +671,7 @@ int msbg_mac_sample(msbg_grid *const *vel, const float *d_pos, int64_t n, int in
     MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_out)), MSBG_EINVAL, "mac sample: null argument");
     if (n == 0) return MSBG_OK;
     for (int a = 0; a < 3; ++a)
-        hipLaunchKernelGGL(sample_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[a]->view(), d_pos, n,
-                           interp, 1, bc, a, d_out + a, 3);
+        launch_sample(vel[a]->view(), d_pos, n, interp, 1, bc, a, d_out + a, 3);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;
@@ -645,6 +683,32 @@ int msbg_g2p_blend(const float *d_u_old, const float *d_u, const float *d_u_tild
     if (n == 0) return MSBG_OK;
     hipLaunchKernelGGL(blend_kernel, dim3(grid_for(3 * n, 256)), dim3(256), 0, stream(), d_u_old, d_u, d_u_tilde, 3 * n,
                        alpha, d_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
+}
+
+int msbg_g2p_flip(msbg_grid *const *vel, msbg_grid *const *vel_tilde, const float *d_pos, const float *d_u_old,
+                  int64_t n, int interp, int bc, float alpha, float *d_out) {
+    MSBG_CHECK(vel && vel_tilde, MSBG_EINVAL, "g2p flip: null grids");
+    msbg_grid *all[6] = {vel[0], vel[1], vel[2], vel_tilde[0], vel_tilde[1], vel_tilde[2]};
+    int rc = check_same_dims(all, 6);
+    if (rc) return rc;
+    MSBG_CHECK(interp == 0 || interp == 1, MSBG_EINVAL, "g2p flip: interp is 0 or 1");
+    MSBG_CHECK(bc >= 0 && bc <= 2, MSBG_EINVAL, "g2p flip: bad boundary");
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_u_old && d_out)), MSBG_EINVAL, "g2p flip: null argument");
+    if (n == 0) return MSBG_OK;
+    MacViews u, ut;
+    for (int a = 0; a < 3; ++a) {
+        u.v[a] = vel[a]->view();
+        ut.v[a] = vel_tilde[a]->view();
+    }
+    if (interp == 0)
+        hipLaunchKernelGGL(g2p_flip_kernel<0>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), u, ut, d_pos, d_u_old,
+                           n, bc, alpha, d_out);
+    else
+        hipLaunchKernelGGL(g2p_flip_kernel<1>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), u, ut, d_pos, d_u_old,
+                           n, bc, alpha, d_out);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;

@@ -13,7 +13,8 @@ them with the band Poisson solve:
                    p = 0 outside the band (the air), zero gradient at the domain faces (walls)
     projection     u += grad p (msbg_mac_subtract_gradient with scale -1) over all MAC blocks,
                    so div u = div - A p, the solve's residual, on every band cell
-    G2P            MacSampler on u and on u*, g2p_blend (fluid/g2p.rs:11-45)
+    G2P            MacSampler on u and on u* and g2p_blend, per particle in one pass
+                   (msbg_g2p_flip, fluid/g2p.rs:11-45)
 
 Every stage runs in the HIP library; the step returns the new particle
 velocities (in Morton order, with the sorted positions) and per-stage
@@ -22,9 +23,9 @@ HIP-event times.
 import numpy as np
 
 from . import lib as _lib
-from .device import BoundaryCondition, DeviceGrid
+from .device import DeviceGrid
 from .solve import PoissonBand, Storage
-from .transfer import MacGrid, MacSampler, g2p_blend, particles_to_grid
+from .transfer import MacGrid, g2p_flip, particles_to_grid
 
 
 class FlipStep:
@@ -76,9 +77,7 @@ class FlipStep:
         rec(4)
         self.mac.subtract_gradient(layout, p, active, -1.0)
         rec(5)
-        u = MacSampler(self.mac, BoundaryCondition.Clamp).sample(ps)
-        u_tilde = MacSampler(self.mac_tilde, BoundaryCondition.Clamp).sample(ps)
-        out = g2p_blend(vs, u, u_tilde, alpha, lib=lib)
+        out = g2p_flip(self.mac, self.mac_tilde, ps, vs, alpha)
         rec(6)
         lib.synchronize()
         names = ("sort", "p2g", "divergence", "solve", "projection", "g2p")

@@ -73,6 +73,7 @@ _SIGS = {
     "msbg_gather_rows": [vp, vp, i64, i64, vp],
     "msbg_sample": [vp, vp, i64, i32, i32, i32, vp],
     "msbg_mac_sample": [vp, vp, i64, i32, i32, vp],
+    "msbg_g2p_flip": [vp, vp, vp, vp, i64, i32, i32, f32, vp],
     "msbg_g2p_blend": [vp, vp, vp, i64, f32, vp],
     "msbg_particles_to_grid_mac": [vp, vp, vp, vp, vp, i64, f32, C.POINTER(C.c_int64)],
     "msbg_p2g_bspline": [i32, vp, vp, vp, vp, vp, i64, C.POINTER(C.c_int64)],

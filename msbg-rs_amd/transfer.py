@@ -104,6 +104,19 @@ def g2p_blend(u_old, u, u_tilde, alpha, lib=None) -> DeviceArray:
     return out
 
 
+def g2p_flip(mac: MacGrid, mac_tilde: MacGrid, pos, u_old, alpha, interp=Interpolation.Linear,
+             bc=BoundaryCondition.Clamp) -> DeviceArray:
+    """fluid/g2p.rs:11-45 per particle: the samples of u (mac) and u*
+    (mac_tilde) and the FLIP/PIC blend, in one pass (the bits of
+    MacSampler.sample twice and g2p_blend)."""
+    lib = mac.lib
+    p, uo = _dev(lib, pos), _dev(lib, u_old)
+    out = lib.empty((p.shape[0], 3), np.float32)
+    lib.call("msbg_g2p_flip", _handles(mac.velocity), _handles(mac_tilde.velocity), p.ptr, uo.ptr, p.shape[0],
+             int(interp), int(bc), alpha, out.ptr)
+    return out
+
+
 class TransferGrid:
     """Cell-centred mass + velocity channels for the B-spline P2G / G2P."""
 

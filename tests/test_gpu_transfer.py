@@ -6,6 +6,7 @@ import pytest
 import test_oracle_goldens as G
 from msbg_rs_amd import transfer as T
 from test_gpu_sweep import assert_bitexact, random_grid
+from test_oracle_transfer import mac_channels
 from transfer_helpers import dense_from_slabs, oracle_g2p_bspline, oracle_p2g_bspline, oracle_p2g_mac
 
 pytestmark = pytest.mark.gpu
@@ -215,3 +216,27 @@ def test_mac_divergence_bitexact(pkg, orc, hip, s, frac):
     assert np.array_equal(got.b2s >= 2, want.b2s >= 2)
     for b in act:
         assert_bitexact(got.slab[got.b2s[b]], want.slab[want.b2s[b]], "div block %d" % b)
+
+
+@pytest.mark.parametrize("interp,bc", [(0, 0), (1, 0), (0, 1)])
+def test_g2p_flip_matches_sample_twice_and_blend(pkg, orc, hip, interp, bc):
+    """msbg_g2p_flip: bit for bit the oracle's two MAC samples and blend."""
+    s = (40, 33, 48)
+    f = lambda a, X, Y, Z: np.sin(0.21 * X + a) * np.cos(0.13 * Y) + 0.03 * Z
+    g = lambda a, X, Y, Z: np.cos(0.17 * X) * np.sin(0.09 * Y + a) - 0.02 * Z
+    from test_gpu_flip import mac_from_channels
+    cu, cut = mac_channels(pkg, s, f), mac_channels(pkg, s, g)  # every block: one slot table for all channels
+    mu, mut = mac_from_channels(pkg, hip, cu), mac_from_channels(pkg, hip, cut)
+    rng = np.random.default_rng(9)
+    n = 20000
+    pos = rng.uniform(-1.0, np.array(s) + 1.0, size=(n, 3)).astype(np.float32)  # some outside the domain
+    uo = rng.normal(size=(n, 3)).astype(np.float32)
+    for alpha in (0.0, 0.95, 1.0):
+        got = T.g2p_flip(mu, mut, pos, uo, alpha, interp, bc).get()
+        su, st = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        for c, out in ((cu, su), (cut, st)):
+            slabs = np.ascontiguousarray(np.stack([np.ascontiguousarray(x.slab) for x in c]))
+            orc.orc_mac_sample(*s, slabs, slabs.shape[1], c[0].b2s, pos, n, interp, bc, out)
+        want = np.zeros_like(uo)
+        orc.orc_g2p_blend(uo, su, st, n, alpha, want)
+        assert_bitexact(got, want, "g2p flip")
