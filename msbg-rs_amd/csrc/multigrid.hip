@@ -881,11 +881,11 @@ int ensure(msbg_poisson *p) {
 
 int n_levels(const msbg_poisson *p) { return p->mg ? (int)p->mg->lv.size() + 1 : 0; }
 
-void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu) {
+void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu, bool first_done) {
     const Hierarchy &H = *p->mg;
     const i64 na = p->n_active;
     const int nl = (int)H.lv.size();
-    pband::rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, nu, false);
+    pband::rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, nu, false, first_done);
     pband::residual_field(p, s, storage, p->d_e, p->d_rt, p->d_r);
     restrict0_kernel<<<grid_of(na * 512), kT, 0, s>>>(na, p->d_blocks, p->dom, p->d_r, H.lv[0].b);
     for (int k = 0; k + 1 < nl; ++k) {

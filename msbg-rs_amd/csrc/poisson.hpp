@@ -137,9 +137,10 @@ void residual(msbg_poisson *p, hipStream_t s, int storage, const void *x, const 
 int read_norms(msbg_poisson *p, double *r2, double *b2);
 // r = b - A x over the listed cells (FP32), without the norm reduction.
 void residual_field(msbg_poisson *p, hipStream_t s, int storage, const void *x, const void *b, float *r);
-// `sweeps` red-black sweeps in place, colour 0 then 1 (reverse: 1 then 0).
+// `sweeps` red-black sweeps in place, colour 0 then 1 (reverse: 1 then 0);
+// skip_first: the first half sweep is already done.
 void rb_sweeps(msbg_poisson *p, hipStream_t s, int storage, void *x, const void *b, float omega, int sweeps,
-               bool reverse);
+               bool reverse, bool skip_first = false);
 
 }  // namespace pband
 
@@ -150,9 +151,10 @@ int ensure(msbg_poisson *p);
 void release(msbg_poisson *p);
 // Levels including the band itself (0 before ensure).
 int n_levels(const msbg_poisson *p);
-// One V-cycle on A e = rt (p->d_e, p->d_rt in `storage`, e = 0 on entry):
-// nu pre- and post-smoothing red-black sweeps per level; p->d_r is scratch.
-void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu);
+// One V-cycle on A e = rt (p->d_e, p->d_rt in `storage`, e = 0 on entry, or
+// e after the first red half sweep when first_done): nu pre- and
+// post-smoothing red-black sweeps per level; p->d_r is scratch.
+void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu, bool first_done);
 
 }  // namespace mg
 

@@ -291,13 +291,22 @@ __global__ __launch_bounds__(256) void reduce_norms_kernel(const double2 *__rest
     }
 }
 
+// One relaxation of a cell whose value and neighbours are all +0 (b: its
+// right-hand side as stored), in the relaxation's operation order.
+__device__ __forceinline__ float first_half(float b, float w6) {
+    const float s = ((((0.0f + 0.0f) + 0.0f) + 0.0f) + 0.0f) + 0.0f;
+    return __builtin_fmaf(w6, __builtin_fmaf(-6.0f, 0.0f, b + s), 0.0f);
+}
+
 // Refinement step start: rt = T(r / scale * extra), e = 0 on the listed cells
-// (extra: an exact power of two)
+// (extra: an exact power of two).  FIRST: e also takes the first red half
+// sweep from e = 0, computed as the relaxation computes it (every neighbour
+// +0): colour 0 gets fmaf(w6, fmaf(-6, 0, rt + 0), 0), colour 1 stays +0.
 // (column mapping of the relaxation, row vectors where the row is whole).
-template <int ST, int Z>
+template <int ST, int Z, bool FIRST>
 __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__restrict__ blocks, Dom d, const float *r,
                                                             const float *scale, float extra, typename Sto<ST>::T *rt,
-                                                            typename Sto<ST>::T *e) {
+                                                            typename Sto<ST>::T *e, float w6) {
     constexpr int W = ColMap<ST, Z>::W;
     const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
@@ -321,12 +330,20 @@ __global__ __launch_bounds__(kThreads) void ir_scale_kernel(const PBlock *__rest
                 v.set(k + 2, q.z * inv);
                 v.set(k + 3, q.w * inv);
             }
+            if (FIRST) {
+                const int par = (Q.gx0 + Q.gy + Q.gz0 + j) & 1;
+#pragma unroll
+                for (int i = 0; i < W; ++i)
+                    if (((par + i) & 1) == 0) z.set(i, first_half(v.at(i), w6));
+            }
             v.store(rt, cj);
             z.store(e, cj);
         } else {
+            const int par = (Q.gx0 + Q.gy + Q.gz0 + j) & 1;
             for (int i = 0; i < Q.n; ++i) {
-                rt[cj + i] = Sto<ST>::cv(r[cj + i] * inv);
-                e[cj + i] = Sto<ST>::cv(0.0f);
+                const typename Sto<ST>::T t = Sto<ST>::cv(r[cj + i] * inv);
+                rt[cj + i] = t;
+                e[cj + i] = Sto<ST>::cv(FIRST && ((par + i) & 1) == 0 ? first_half(Sto<ST>::ld(&t, 0), w6) : 0.0f);
             }
         }
     }
@@ -490,12 +507,16 @@ void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, f
 }
 
 template <int ST>
-void ir_scale_st(msbg_poisson *p, hipStream_t s, float extra) {
+void ir_scale_st(msbg_poisson *p, hipStream_t s, float extra, float w6) {
     using T = typename Sto<ST>::T;
     with_z<ST>(p, [&](auto zc) {
         constexpr int Z = decltype(zc)::value;
-        ir_scale_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom, p->d_r, p->d_scale, extra,
-                                                                          (T *)p->d_rt, (T *)p->d_e);
+        if (w6 > 0.0f)
+            ir_scale_kernel<ST, Z, true><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(
+                p->d_blocks, p->dom, p->d_r, p->d_scale, extra, (T *)p->d_rt, (T *)p->d_e, w6);
+        else
+            ir_scale_kernel<ST, Z, false><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(
+                p->d_blocks, p->dom, p->d_r, p->d_scale, extra, (T *)p->d_rt, (T *)p->d_e, 0.0f);
     });
 }
 
@@ -509,13 +530,14 @@ void ir_update_st(msbg_poisson *p, hipStream_t s, float *x, float extra) {
     });
 }
 
-void ir_scale(msbg_poisson *p, hipStream_t s, int st, float extra) {
+// w6 > 0: the first red half sweep (factor w6 = omega / 6) is folded in.
+void ir_scale(msbg_poisson *p, hipStream_t s, int st, float extra, float w6) {
     if (st == MSBG_STORAGE_F32)
-        ir_scale_st<MSBG_STORAGE_F32>(p, s, extra);
+        ir_scale_st<MSBG_STORAGE_F32>(p, s, extra, w6);
     else if (st == MSBG_STORAGE_F16)
-        ir_scale_st<MSBG_STORAGE_F16>(p, s, extra);
+        ir_scale_st<MSBG_STORAGE_F16>(p, s, extra, w6);
     else
-        ir_scale_st<MSBG_STORAGE_BF16>(p, s, extra);
+        ir_scale_st<MSBG_STORAGE_BF16>(p, s, extra, w6);
 }
 
 void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x, float extra) {
@@ -570,11 +592,13 @@ void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, con
         // leaving 2^12 * 65504 of headroom (entries below rms / 4 become
         // subnormal, an absolute error <= 2^-13 rms).
         const float extra = (mode == 2 && storage == MSBG_STORAGE_F16) ? 0x1p-12f : 1.0f;
-        ir_scale(p, s, storage, extra);
+        // the first red half sweep from e = 0 is computed by the scale kernel
+        const bool fold = sweeps > 0;
+        ir_scale(p, s, storage, extra, fold ? omega / 6.0f : 0.0f);
         if (mode == 2)
-            mg::vcycle(p, s, storage, omega, sweeps);
+            mg::vcycle(p, s, storage, omega, sweeps, fold);
         else
-            rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps);
+            pband::rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps, false, fold);
         ir_update(p, s, storage, x, extra);
         residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
     }
@@ -633,9 +657,9 @@ void residual_field(msbg_poisson *p, hipStream_t s, int st, const void *x, const
 }
 
 void rb_sweeps(msbg_poisson *p, hipStream_t s, int st, void *x, const void *b, float omega, int sweeps,
-               bool reverse) {
+               bool reverse, bool skip_first) {
     for (int i = 0; i < sweeps; ++i) {
-        relax(p, s, st, x, x, b, omega, reverse ? 1 : 0, true);
+        if (!(skip_first && i == 0)) relax(p, s, st, x, x, b, omega, reverse ? 1 : 0, true);
         relax(p, s, st, x, x, b, omega, reverse ? 0 : 1, true);
     }
 }
