@@ -293,77 +293,78 @@ __device__ __forceinline__ float level1_at(const float *__restrict__ x1, const i
     return x1[nbk * 512 + (qx & 7) + 8 * (qy & 7) + 64 * (qz & 7)];
 }
 
-// Level 1 -> 0: e += P x1 on the in-domain cells (e in storage ST).  A
-// thread takes 8 x-adjacent fine cells (x = 8 hx .. 8 hx + 7) of one row:
-// their trilinear partners are 6 x-positions (4 hx - 1 .. 4 hx + 4) on 4
-// (y, z) rows of level 1.  The fine row is read and written as 16 / 32 bytes.
+// Level 1 -> 0: e += P x1 on the in-domain cells (e in storage ST).  One
+// work-group per listed block stages the block's level-1 cells with a 1-cell
+// ring (10^3 values, 0 outside the band) in LDS; thread t then interpolates
+// the 16 cells of row (y, z) = (t & 15, t >> 4): their trilinear partners are
+// the level-1 x-positions -1 .. 8 on 4 (y, z) rows.  A partner beyond the
+// domain is the cell itself (clamp), one outside the band reads 0.
 template <int ST>
-__global__ __launch_bounds__(kT) void prolong0_kernel(i64 n_active, const PBlock *__restrict__ blocks, Dom d,
-                                                      const int32_t *__restrict__ nb27, const float *__restrict__ x1,
-                                                      typename Sto<ST>::T *e) {
+__global__ __launch_bounds__(256) void prolong0_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                       const int32_t *__restrict__ nb27, const float *__restrict__ x1,
+                                                       typename Sto<ST>::T *e) {
     using S = Sto<ST>;
-    const i64 t = (i64)blockIdx.x * kT + threadIdx.x;
-    if (t >= n_active * 512) return;
-    const i64 k = t >> 9;
-    const int r = (int)(t & 511), hx = r & 1, cy = (r >> 1) & 15, cz = r >> 5;
+    __shared__ float s_c[1000];
+    const i64 k = blockIdx.x;
     const PBlock B = blocks[k];
-    const int x0 = 8 * hx;
-    if (B.ox + x0 >= d.sx || B.oy + cy >= d.sy || B.oz + cz >= d.sz) return;
     const int32_t *T = nb27 + k * 27;
-    // y / z partner offsets (0: clamped at the domain boundary)
+    for (int s = threadIdx.x; s < 1000; s += 256)
+        s_c[s] = level1_at(x1, T, k, s % 10 - 1, (s / 10) % 10 - 1, s / 100 - 1);
+    __syncthreads();
+    const int cy = threadIdx.x & 15, cz = threadIdx.x >> 4;
+    if (B.oy + cy >= d.sy || B.oz + cz >= d.sz) return;
     const int py = cy >> 1, pz = cz >> 1;
     int dy = (cy & 1) ? 1 : -1, dz = (cz & 1) ? 1 : -1;
     if ((dy > 0 && B.oy + 2 * (py + 1) >= d.sy) || (dy < 0 && py == 0 && T[nb27_at(0, -1, 0)] == -2)) dy = 0;
     if ((dz > 0 && B.oz + 2 * (pz + 1) >= d.sz) || (dz < 0 && pz == 0 && T[nb27_at(0, 0, -1)] == -2)) dz = 0;
-    float v[4][6];
+    const bool left_wall = T[nb27_at(-1, 0, 0)] == -2;
+    const float *rows[4];
 #pragma unroll
     for (int yz = 0; yz < 4; ++yz) {
         const int qy = py + ((yz & 1) ? dy : 0), qz = pz + ((yz & 2) ? dz : 0);
-#pragma unroll
-        for (int jx = 0; jx < 6; ++jx) v[yz][jx] = level1_at(x1, T, k, 4 * hx - 1 + jx, qy, qz);
+        rows[yz] = s_c + (qz + 1) * 100 + (qy + 1) * 10 + 1;  // rows[yz][qx], qx = -1 .. 8
     }
-    const bool left_wall = hx == 0 && T[nb27_at(-1, 0, 0)] == -2;
-    float out[8];
+    float out[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int px = 4 * hx + (i >> 1);
+    for (int i = 0; i < 16; ++i) {
+        const int px = i >> 1;
         int qn = px + ((i & 1) ? 1 : -1);
         if ((qn > px && B.ox + 2 * qn >= d.sx) || (qn < 0 && left_wall)) qn = px;  // clamp
-        const int a = px - (4 * hx - 1), c = qn - (4 * hx - 1);
         float val = 0.0f;
 #pragma unroll
         for (int yz = 0; yz < 4; ++yz) {
             const float wyz = ((yz & 1) ? 0.25f : 0.75f) * ((yz & 2) ? 0.25f : 0.75f);
-            val += wyz * (0.75f * v[yz][a] + 0.25f * v[yz][c]);
+            val += wyz * (0.75f * rows[yz][px] + 0.25f * rows[yz][qn]);
         }
         out[i] = val;
     }
-    const i64 j = ((i64)B.slot << 12) + x0 + 16 * cy + 256 * cz;
-    const int n = min(8, d.sx - B.ox - x0);
-    if (n == 8) {
+    const i64 j = ((i64)B.slot << 12) + 16 * cy + 256 * cz;
+    const int n = min(16, d.sx - B.ox);
+    if (n == 16) {
         if constexpr (ST == MSBG_STORAGE_F32) {
             float4 *q = reinterpret_cast<float4 *>(e + j);
-            float4 a = q[0], c = q[1];
-            a.x += out[0];
-            a.y += out[1];
-            a.z += out[2];
-            a.w += out[3];
-            c.x += out[4];
-            c.y += out[5];
-            c.z += out[6];
-            c.w += out[7];
-            q[0] = a;
-            q[1] = c;
-        } else {
-            uint4 q = *reinterpret_cast<const uint4 *>(e + j);
-            uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const float lo = S::ld(reinterpret_cast<const typename S::T *>(&w[m]), 0) + out[2 * m];
-                const float hi = S::ld(reinterpret_cast<const typename S::T *>(&w[m]), 1) + out[2 * m + 1];
-                w[m] = (uint32_t)S::cv(lo) | (uint32_t)S::cv(hi) << 16;
+                float4 a = q[m];
+                a.x += out[4 * m];
+                a.y += out[4 * m + 1];
+                a.z += out[4 * m + 2];
+                a.w += out[4 * m + 3];
+                q[m] = a;
             }
-            *reinterpret_cast<uint4 *>(e + j) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint4 q = *reinterpret_cast<const uint4 *>(e + j + 8 * h);
+                uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float lo = S::ld(reinterpret_cast<const typename S::T *>(&w[m]), 0) + out[8 * h + 2 * m];
+                    const float hi = S::ld(reinterpret_cast<const typename S::T *>(&w[m]), 1) + out[8 * h + 2 * m + 1];
+                    w[m] = (uint32_t)S::cv(lo) | (uint32_t)S::cv(hi) << 16;
+                }
+                *reinterpret_cast<uint4 *>(e + j + 8 * h) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
         }
     } else {
         for (int i = 0; i < n; ++i) e[j + i] = S::cv(S::ld(e, j + i) + out[i]);
@@ -902,16 +903,14 @@ void vcycle(msbg_poisson *p, hipStream_t s, int storage, float omega, int nu, bo
         prolong_to(p, H.lv[k + 1], L, s);
         smooth(p, L, s, omega, nu, true, false);
     }
-    const int g = grid_of(na * 512);
+    const int g = (int)na;  // a work-group per listed block
     if (storage == MSBG_STORAGE_F32)
-        prolong0_kernel<MSBG_STORAGE_F32><<<g, kT, 0, s>>>(na, p->d_blocks, p->dom, H.nb27, H.lv[0].x,
-                                                           (float *)p->d_e);
+        prolong0_kernel<MSBG_STORAGE_F32><<<g, 256, 0, s>>>(p->d_blocks, p->dom, H.nb27, H.lv[0].x, (float *)p->d_e);
     else if (storage == MSBG_STORAGE_F16)
-        prolong0_kernel<MSBG_STORAGE_F16><<<g, kT, 0, s>>>(na, p->d_blocks, p->dom, H.nb27, H.lv[0].x,
-                                                           (uint16_t *)p->d_e);
+        prolong0_kernel<MSBG_STORAGE_F16><<<g, 256, 0, s>>>(p->d_blocks, p->dom, H.nb27, H.lv[0].x, (uint16_t *)p->d_e);
     else
-        prolong0_kernel<MSBG_STORAGE_BF16><<<g, kT, 0, s>>>(na, p->d_blocks, p->dom, H.nb27, H.lv[0].x,
-                                                            (uint16_t *)p->d_e);
+        prolong0_kernel<MSBG_STORAGE_BF16><<<g, 256, 0, s>>>(p->d_blocks, p->dom, H.nb27, H.lv[0].x,
+                                                             (uint16_t *)p->d_e);
     pband::rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, nu, true);
 }
 
