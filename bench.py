@@ -128,6 +128,46 @@ def particle_leg(pkg, lib, sc, n, steps, warmup):
     return out
 
 
+def halo_leg(lib, fine, coarse, levels, active, steps):
+    """Standalone cross-level halo gather (fill_multires, full tiles, Neumann)
+    of every level-0 block of the 512^3 grid, H = 1 and 2.  Each call is one
+    gather_kernel launch plus the upload of the block list; the HIP events
+    bracket the whole call, so `achieved` is a floor for the kernel.
+    Algorithmic bytes per block: the 16^3 interior read once (16 KiB) plus the
+    (16 + 2H)^3 tile written."""
+    import ctypes as C
+
+    b = np.ascontiguousarray(active, dtype=np.int64)
+    bp = b.ctypes.data_as(C.c_void_p)
+    out = {"blocks": int(b.size)}
+    for halo in (1, 2):
+        hsx = 16 + 2 * halo
+        tiles = lib.empty((b.size, hsx, hsx, hsx), np.float32)
+
+        def call():
+            lib.call("msbg_halo_gather_multires", fine.h, coarse.h, levels.ptr, bp, b.size, halo, 1, 0, tiles.ptr)
+
+        for _ in range(2):
+            call()
+        e0, e1 = lib.event(), lib.event()
+        ms = 0.0
+        for _ in range(steps):
+            lib.call("msbg_event_record", e0)
+            call()
+            lib.call("msbg_event_record", e1)
+            ms += lib.elapsed_ms(e0, e1)
+        ms /= steps
+        lib.call("msbg_event_destroy", e0)
+        lib.call("msbg_event_destroy", e1)
+        alg = b.size * (4096 * 4 + hsx ** 3 * 4)
+        gbs = alg / (ms * 1e-3) / 1e9
+        out["halo%d" % halo] = {"ms_per_call": ms, "algorithmic_bytes": alg, "achieved_GBs": gbs,
+                                "frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
+                                "tile_cells_per_s": b.size * hsx ** 3 / (ms * 1e-3)}
+        tiles.free()
+    return out
+
+
 def transfer_timing(lib, dims, pos, vel, order, cap, steps, warmup):
     """Morton sort, B-spline P2G and G2P of resident particles, each timed with
     a HIP event pair around its call."""
@@ -166,6 +206,15 @@ def transfer_timing(lib, dims, pos, vel, order, cap, steps, warmup):
     for key in ms:
         ms[key] /= steps
     step_ms = ms["p2g"] + ms["g2p"]
+    # Algorithmic bytes per call (each input read once, each output written
+    # once).  P2G call = Morton key (20 B) + radix sort (3 x 16 B) + payload
+    # permute (2 x 24 B) + pos / vel read (24 B) per particle, and 16 B (m and
+    # three momenta) per active cell written.  G2P call = its own key + sort
+    # (68 B) + pos read and velocity written (24 B) per particle, and 12 B per
+    # active cell read.
+    cells = k.value * 4096
+    p2g_bytes = 140 * n + 16 * cells
+    g2p_bytes = 92 * n + 12 * cells
     return {
         "particles": n,
         "order": order,
@@ -173,6 +222,9 @@ def transfer_timing(lib, dims, pos, vel, order, cap, steps, warmup):
         "ms": ms,
         "morton_sort_particles_per_s": n / (ms["sort"] * 1e-3),
         "p2g_g2p_particles_per_s": n / (step_ms * 1e-3),
+        "p2g_algorithmic_GBs": p2g_bytes / (ms["p2g"] * 1e-3) / 1e9,
+        "g2p_algorithmic_GBs": g2p_bytes / (ms["g2p"] * 1e-3) / 1e9,
+        "p2g_g2p_frac_of_hbm_peak": (p2g_bytes + g2p_bytes) / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
     }
 
 
@@ -459,6 +511,8 @@ def main():
     ap.add_argument("--particles", type=int, default=16 * 1024 * 1024)
     ap.add_argument("--particle-steps", type=int, default=5)
     ap.add_argument("--shell-particles", type=int, default=1 << 20)
+    ap.add_argument("--halo-steps", type=int, default=20,
+                    help="timed calls of the standalone 512^3 halo gather leg (0 = skip)")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--only-smoother", action="store_true",
                     help="time the smoother step only (for counter runs): no particle, shell, solve or CPU legs")
@@ -474,6 +528,8 @@ def main():
                     help="domain edge of the FLIP-step leg (dam-break box + droplets; 0 = skip)")
     ap.add_argument("--flip-particles", type=int, default=64 << 20)
     ap.add_argument("--flip-sweeps", type=int, default=6000, help="sweep cap of the FLIP pressure solve")
+    ap.add_argument("--skip-solves", action="store_true",
+                    help="stop after the particle leg: no 128^3 shell, 512^3 band-solve or CPU legs (kernel profiles)")
     ap.add_argument("--fp64-band", action="store_true",
                     help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
     args = ap.parse_args()
@@ -582,6 +638,9 @@ def main():
     if args.only_smoother:
         print(json.dumps(out), flush=True)
         return
+    if args.halo_steps > 0:
+        log("halo gather leg ...")
+        out["halo_gather"] = halo_leg(lib, fine, coarse, levels, active, args.halo_steps)
     if args.scale_up > 0:
         log("scale-up leg ...")
         out["scale_up"] = scale_up_leg(pkg, lib, args.scale_up, 3)
@@ -598,6 +657,9 @@ def main():
         log("particle leg ...")
         out["particles"] = particle_leg(pkg, lib, sc, args.particles, args.particle_steps, 2)
     from msbg_rs_amd import scenes
+    if args.skip_solves:
+        print(json.dumps(out), flush=True)
+        return
     log("128^3 shell legs ...")
     out["shell_128"], (g1, bids1, xs1) = shell_legs(pkg, lib, args.shell_particles, 20, 3, args.solve_reps)
     log("512^3 band solve ...")

@@ -12,6 +12,7 @@
 //    voxel; every cell gathers its 27 neighbour voxels in a fixed order, so
 //    the result is deterministic and needs neither atomics nor colouring.
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "primitives.hpp"
@@ -515,6 +516,7 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
                                                           const float4 *__restrict__ pm,
                                                           const float4 *__restrict__ pu) {
     __shared__ int s_nb[27];
+    __shared__ uint2 s_rng[9][256];  // per-lane voxel ranges of the current plane (each lane its own column)
     const int64_t bid = active[blockIdx.x];
     const int bx = (int)(bid % mg.nx), by = (int)((bid / mg.nx) % mg.ny), bz = (int)(bid / ((int64_t)mg.nx * mg.ny));
     if (threadIdx.x < 27) {
@@ -542,24 +544,42 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
     for (int lv = -1; lv <= 16; ++lv) {
         const int vz = cz0 + lv;
         if (vz >= 0 && vz < mg.sz) {
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int vy = cy + dy;
-                if (vy < 0 || vy >= mg.sy) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int vx = cx + dx;
-                    if (vx < 0 || vx >= mg.sx) continue;
+            // The plane's 9 voxel ranges of this lane, (dy, dx) in order; an
+            // outside or absent voxel is an empty range.
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int vy = cy + t / 3 - 1, vx = cx + t % 3 - 1;
+                uint2 r = make_uint2(0u, 0u);
+                if (vy >= 0 && vy < mg.sy && vx >= 0 && vx < mg.sx) {
                     const int slot = s_nb[((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1)];
-                    if (slot < 2) continue;
-                    const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
-                    const uint32_t e = vend[c];
-                    for (uint32_t j = vbeg[c]; j < e; ++j) {
-                        const float4 p = pm[j];
-                        const float4 u = pu[j];
-                        const float wxy = bspline_w<ORDER>(p.x, cx) * bspline_w<ORDER>(p.y, cy);
-                        if (lv >= 1) acc_add<ORDER>(lo, wxy, p.z, cz0 + lv - 1, p.w, u);
-                        if (lv >= 0 && lv <= 15) acc_add<ORDER>(mid, wxy, p.z, cz0 + lv, p.w, u);
-                        if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
+                    if (slot >= 2) {
+                        const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
+                        r = make_uint2(vbeg[c], vend[c]);
                     }
+                }
+                s_rng[t][threadIdx.x] = r;
+            }
+            // One pass over the concatenation of the 9 ranges: a wave steps
+            // as often as its fullest lane has particles, not once per voxel
+            // per the fullest lane of that voxel.  Per cell, the particles
+            // still arrive voxel by voxel in (dy, dx) order, sorted within.
+            int t = 0;
+            uint32_t j = s_rng[0][threadIdx.x].x, e = s_rng[0][threadIdx.x].y;
+            while (j >= e && ++t < 9) {
+                j = s_rng[t][threadIdx.x].x;
+                e = s_rng[t][threadIdx.x].y;
+            }
+            while (t < 9) {
+                const float4 p = pm[j];
+                const float4 u = pu[j];
+                const float wxy = bspline_w<ORDER>(p.x, cx) * bspline_w<ORDER>(p.y, cy);
+                if (lv >= 1) acc_add<ORDER>(lo, wxy, p.z, cz0 + lv - 1, p.w, u);
+                if (lv >= 0 && lv <= 15) acc_add<ORDER>(mid, wxy, p.z, cz0 + lv, p.w, u);
+                if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
+                ++j;
+                while (j >= e && ++t < 9) {
+                    j = s_rng[t][threadIdx.x].x;
+                    e = s_rng[t][threadIdx.x].y;
                 }
             }
         }
@@ -570,40 +590,172 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
     }
 }
 
-template <int ORDER>
-__global__ void g2p_bspline_kernel(GridView v0, GridView v1, GridView v2, const float *__restrict__ pos, int64_t n,
-                                   const uint32_t *__restrict__ visit, float *__restrict__ out) {
-    // Lane k handles particle visit[k] (Morton order), so neighbouring lanes
-    // read neighbouring grid cells; each particle's sum is independent of it.
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = visit[k];
-        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-        const int fx = (int)floorf(px), fy = (int)floorf(py), fz = (int)floorf(pz);
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int cz = fz + dz;
-            if (cz < 0 || cz >= v0.sz) continue;
-            const float wz = bspline_w<ORDER>(pz, cz);
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int cy = fy + dy;
-                if (cy < 0 || cy >= v0.sy) continue;
-                const float wy = bspline_w<ORDER>(py, cy);
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int cx = fx + dx;
-                    if (cx < 0 || cx >= v0.sx) continue;
-                    const float w = (bspline_w<ORDER>(px, cx) * wy) * wz;
-                    const int64_t bid = (cx >> 4) + (int64_t)(cy >> 4) * v0.nx + (int64_t)(cz >> 4) * v0.nx * v0.ny;
+// G2P work-groups: 256 consecutive Morton-sorted particles each.  The grid
+// cells their supports reach (the chunk's box of floor(p) -1 .. +1, clipped
+// to the domain) are staged in LDS when the box holds at most kG2pWindow
+// cells; a chunk with a larger box (it straddles a coarse Morton boundary)
+// reads the slabs directly.  Both paths take the same values in the same
+// order, so the result does not depend on which one runs.
+constexpr int kG2pThreads = 256;
+constexpr int kG2pWindow = 3072;
+constexpr int kG2pWindowBlocks = 64;  // 16^3 blocks a staged box may touch
+
+// floor(p) as a cell index, held to [-2, s + 1]: beyond that every support
+// cell floor(p) - 1 .. floor(p) + 1 lies outside the domain either way, and
+// the float -> int conversion stays in range (NaN maps to -2: no cells).
+__device__ __forceinline__ int floor_cell(float p, int s) {
+    return (int)fminf(fmaxf(floorf(p), -2.0f), (float)(s + 1));
+}
+
+// Cell range [lo, hi] of one axis of a particle's support, clipped to the
+// domain (empty when lo > hi).
+__device__ __forceinline__ void support_range(int f, int s, int &lo, int &hi) {
+    lo = max(f - 1, 0);
+    hi = min(f + 1, s - 1);
+}
+
+// One particle's sum over its 27 support cells (z, then y, then x), the
+// oracle's order.  STAGED reads the cells from the LDS window `win` (box
+// origin x0, y0, z0, extents ex, ey), else straight from the slabs.
+template <int ORDER, bool STAGED>
+__device__ __forceinline__ void g2p_sum(const GridView &v0, const GridView &v1, const GridView &v2, float px,
+                                        float py, float pz, int fx, int fy, int fz, const float (*win)[kG2pWindow],
+                                        int x0, int y0, int z0, int ex, int ey, float &a0, float &a1, float &a2) {
+    a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int cz = fz + dz;
+        if (cz < 0 || cz >= v0.sz) continue;
+        const float wz = bspline_w<ORDER>(pz, cz);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int cy = fy + dy;
+            if (cy < 0 || cy >= v0.sy) continue;
+            const float wy = bspline_w<ORDER>(py, cy);
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = fx + dx;
+                if (cx < 0 || cx >= v0.sx) continue;
+                const float w = (bspline_w<ORDER>(px, cx) * wy) * wz;
+                float s0, s1, s2;
+                if constexpr (STAGED) {  // every in-domain support cell lies in the box
+                    const int c = ((cz - z0) * ey + (cy - y0)) * ex + (cx - x0);
+                    s0 = win[0][c];
+                    s1 = win[1][c];
+                    s2 = win[2][c];
+                } else {
+                    const int64_t bid =
+                        (cx >> 4) + (int64_t)(cy >> 4) * v0.nx + (int64_t)(cz >> 4) * v0.nx * v0.ny;
                     const int vid = (cx & 15) | ((cy & 15) << 4) | ((cz & 15) << 8);
-                    a0 = a0 + w * v0.slab[(int64_t)v0.b2s[bid] * 4096 + vid];
-                    a1 = a1 + w * v1.slab[(int64_t)v1.b2s[bid] * 4096 + vid];
-                    a2 = a2 + w * v2.slab[(int64_t)v2.b2s[bid] * 4096 + vid];
+                    s0 = v0.slab[(int64_t)v0.b2s[bid] * 4096 + vid];
+                    s1 = v1.slab[(int64_t)v1.b2s[bid] * 4096 + vid];
+                    s2 = v2.slab[(int64_t)v2.b2s[bid] * 4096 + vid];
                 }
+                a0 = a0 + w * s0;
+                a1 = a1 + w * s1;
+                a2 = a2 + w * s2;
             }
         }
-        out[3 * i] = a0;
-        out[3 * i + 1] = a1;
-        out[3 * i + 2] = a2;
     }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kG2pThreads) void g2p_bspline_kernel(GridView v0, GridView v1, GridView v2,
+                                                                  const float *__restrict__ pos, int64_t n,
+                                                                  const uint32_t *__restrict__ visit,
+                                                                  float *__restrict__ out) {
+    __shared__ float win[3][kG2pWindow];
+    __shared__ int s_box[6];                    // min x, y, z; max x, y, z of the chunk's support cells
+    __shared__ int s_slot[3][kG2pWindowBlocks];  // slab slot of each block the box touches, per component
+    // Consecutive chunks (neighbouring cells) land on one XCD and share its L2.
+    const int64_t k = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kG2pThreads + threadIdx.x;
+    if (threadIdx.x < 3) s_box[threadIdx.x] = INT_MAX;
+    else if (threadIdx.x < 6) s_box[threadIdx.x] = INT_MIN;
+    int64_t i = -1;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    int fx = 0, fy = 0, fz = 0;
+    // support box of this lane, inverted (no cells) unless it reaches the domain
+    int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
+    if (k < n) {
+        i = visit[k];
+        px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        fx = floor_cell(px, v0.sx), fy = floor_cell(py, v0.sy), fz = floor_cell(pz, v0.sz);
+        int ax, bx, ay, by, az, bz;
+        support_range(fx, v0.sx, ax, bx);
+        support_range(fy, v0.sy, ay, by);
+        support_range(fz, v0.sz, az, bz);
+        if (ax <= bx && ay <= by && az <= bz) lx = ax, hx = bx, ly = ay, hy = by, lz = az, hz = bz;
+    }
+    // wave-level min / max first: one LDS atomic per wave and bound
+    for (int o = 32; o > 0; o >>= 1) {
+        lx = min(lx, __shfl_xor(lx, o)), ly = min(ly, __shfl_xor(ly, o)), lz = min(lz, __shfl_xor(lz, o));
+        hx = max(hx, __shfl_xor(hx, o)), hy = max(hy, __shfl_xor(hy, o)), hz = max(hz, __shfl_xor(hz, o));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && lx <= hx) {
+        atomicMin(&s_box[0], lx);
+        atomicMin(&s_box[1], ly);
+        atomicMin(&s_box[2], lz);
+        atomicMax(&s_box[3], hx);
+        atomicMax(&s_box[4], hy);
+        atomicMax(&s_box[5], hz);
+    }
+    __syncthreads();
+    const int x0 = s_box[0], y0 = s_box[1], z0 = s_box[2];
+    // box extents; none of the chunk's lanes reaching the domain leaves the box inverted (ex <= 0)
+    const int64_t ex = (int64_t)s_box[3] - x0 + 1, ey = (int64_t)s_box[4] - y0 + 1, ez = (int64_t)s_box[5] - z0 + 1;
+    const int bx0 = x0 >> 4, by0 = y0 >> 4, bz0 = z0 >> 4;
+    const int64_t nbx = ex > 0 ? ((s_box[3] >> 4) - bx0 + 1) : 0, nby = ex > 0 ? ((s_box[4] >> 4) - by0 + 1) : 0,
+                  nbz = ex > 0 ? ((s_box[5] >> 4) - bz0 + 1) : 0;
+    float a0, a1, a2;
+    // uniform over the work-group
+    if (ex > 0 && ex * ey * ez <= kG2pWindow && nbx * nby * nbz <= kG2pWindowBlocks) {
+        const int nbxy = (int)(nbx * nby), nb = (int)(nbxy * nbz);
+        if (threadIdx.x < nb) {
+            const int t = threadIdx.x;
+            const int64_t bid = (bx0 + t % (int)nbx) + (int64_t)(by0 + (t % nbxy) / (int)nbx) * v0.nx +
+                                (int64_t)(bz0 + t / nbxy) * v0.nx * v0.ny;
+            s_slot[0][t] = v0.b2s[bid];
+            s_slot[1][t] = v1.b2s[bid];
+            s_slot[2][t] = v2.b2s[bid];
+        }
+        __syncthreads();
+        // Each lane's cells are addressed first and loaded back to back.
+        constexpr int ITEMS = kG2pWindow / kG2pThreads;
+        const int exy = (int)(ex * ey), nw = (int)(exy * ez);
+        int64_t off[3][ITEMS];
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            const int c = threadIdx.x + q * kG2pThreads;
+            off[0][q] = off[1][q] = off[2][q] = -1;
+            if (c < nw) {
+                const int cz = z0 + c / exy, cy = y0 + (c % exy) / (int)ex, cx = x0 + c % (int)ex;
+                const int b = ((cx >> 4) - bx0) + (int)nbx * ((cy >> 4) - by0) + nbxy * ((cz >> 4) - bz0);
+                const int vid = (cx & 15) | ((cy & 15) << 4) | ((cz & 15) << 8);
+                off[0][q] = (int64_t)s_slot[0][b] * 4096 + vid;
+                off[1][q] = (int64_t)s_slot[1][b] * 4096 + vid;
+                off[2][q] = (int64_t)s_slot[2][b] * 4096 + vid;
+            }
+        }
+        float val[3][ITEMS];
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            val[0][q] = off[0][q] >= 0 ? v0.slab[off[0][q]] : 0.0f;
+            val[1][q] = off[1][q] >= 0 ? v1.slab[off[1][q]] : 0.0f;
+            val[2][q] = off[2][q] >= 0 ? v2.slab[off[2][q]] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q) {
+            const int c = threadIdx.x + q * kG2pThreads;
+            if (c < nw) win[0][c] = val[0][q], win[1][c] = val[1][q], win[2][c] = val[2][q];
+        }
+        __syncthreads();
+        if (k >= n) return;
+        g2p_sum<ORDER, true>(v0, v1, v2, px, py, pz, fx, fy, fz, win, x0, y0, z0, (int)ex, (int)ey, a0, a1, a2);
+    } else {
+        if (k >= n) return;
+        g2p_sum<ORDER, false>(v0, v1, v2, px, py, pz, fx, fy, fz, win, 0, 0, 0, 0, 0, a0, a1, a2);
+    }
+    out[3 * i] = a0;
+    out[3 * i + 1] = a1;
+    out[3 * i + 2] = a2;
 }
 
 // Ensures `bids` (host) on every grid and zero-fills them (MacGrid::zero_blocks,
@@ -896,12 +1048,14 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
     uint32_t *visit = keys + n;
     rc = msbg_morton_sort(d_pos, n, keys, visit);
     if (rc) return rc;
+    // one lane per particle (n < 2^31, so the chunk count fits the grid)
+    const dim3 chunks((unsigned)((n + kG2pThreads - 1) / kG2pThreads)), lanes(kG2pThreads);
     if (order == 1)
-        hipLaunchKernelGGL(g2p_bspline_kernel<1>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
-                           vel[1]->view(), vel[2]->view(), d_pos, n, visit, d_out);
+        hipLaunchKernelGGL(g2p_bspline_kernel<1>, chunks, lanes, 0, stream(), vel[0]->view(), vel[1]->view(),
+                           vel[2]->view(), d_pos, n, visit, d_out);
     else
-        hipLaunchKernelGGL(g2p_bspline_kernel<2>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vel[0]->view(),
-                           vel[1]->view(), vel[2]->view(), d_pos, n, visit, d_out);
+        hipLaunchKernelGGL(g2p_bspline_kernel<2>, chunks, lanes, 0, stream(), vel[0]->view(), vel[1]->view(),
+                           vel[2]->view(), d_pos, n, visit, d_out);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;

@@ -161,6 +161,28 @@ def test_bspline_p2g_g2p_bitexact(pkg, orc, hip, order, n, s):
     assert_bitexact(out, want, "g2p")
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_bspline_g2p_staged_and_direct_chunks(pkg, orc, hip, order):
+    """G2P chunks of 256 sorted particles whose support box fits the LDS
+    window (a dense cluster) and chunks whose box does not (sparse particles
+    over the whole domain), with particles partly or wholly outside it."""
+    s = (96, 80, 112)
+    rng = np.random.default_rng(7 + order)
+    src = rng.uniform(0.0, np.array(s) - 1e-3, size=(60000, 3)).astype(np.float32)
+    vel = rng.normal(size=src.shape).astype(np.float32)
+    active, b2s, slabs = oracle_p2g_bspline(orc, order, s, src, vel, None)
+    tg = T.TransferGrid(*s, capacity_blocks=active.size + 4, lib=hip)
+    assert tg.p2g(order, src, vel, None) == active.size
+    cluster = rng.normal(loc=[40.0, 30.0, 50.0], scale=3.0, size=(5000, 3))
+    sparse = rng.uniform(-4.0, np.array(s) + 4.0, size=(3000, 3))
+    pos = np.concatenate([cluster, sparse]).astype(np.float32)
+    rng.shuffle(pos)
+    out = tg.g2p(order, pos).get()
+    want = oracle_g2p_bspline(orc, order, s, b2s, slabs[1:], pos)
+    assert np.any(want[:, 0] == 0.0) and np.any(want[:, 0] != 0.0)
+    assert_bitexact(out, want, "g2p")
+
+
 @pytest.mark.parametrize("r_scan,s,n", [(4.0, (32, 32, 32), 1), (4.0, (48, 40, 33), 3000), (2.5, (40, 33, 48), 5000),
                                         (6.0, (64, 64, 48), 2000), (4.0, (128, 128, 128), 200000)])
 def test_density_splat_bitexact(pkg, orc, hip, r_scan, s, n):
