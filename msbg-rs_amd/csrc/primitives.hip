@@ -158,18 +158,21 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t *__restr
                                                          uint32_t *__restrict__ vout, int64_t n, int shift,
                                                          const uint32_t *__restrict__ offsets, int64_t ntiles) {
     __shared__ uint32_t cnt[4][kRsBins];
-    __shared__ uint32_t base[kRsBins];
+    __shared__ uint32_t base[kRsBins];   // first output slot of digit d from this tile
+    __shared__ uint32_t lbase[kRsBins];  // first tile-local slot of digit d
+    __shared__ uint32_t sk[kRsTile], sv[kRsTile];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 #pragma unroll
     for (int w = 0; w < 4; ++w) cnt[w][t] = 0;
     base[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
     __syncthreads();
-    const int64_t tile0 = (int64_t)blockIdx.x * kRsTile + (int64_t)wave * (kRsChunks * 64);
+    const int64_t tile0 = (int64_t)blockIdx.x * kRsTile;
+    const int64_t wave0 = tile0 + (int64_t)wave * (kRsChunks * 64);
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t k[kRsChunks], v[kRsChunks], off[kRsChunks];
 #pragma unroll
     for (int c = 0; c < kRsChunks; ++c) {
-        const int64_t idx = tile0 + c * 64 + lane;
+        const int64_t idx = wave0 + c * 64 + lane;
         const bool valid = idx < n;
         k[c] = valid ? kin[idx] : 0u;
         v[c] = valid ? vin[idx] : 0u;
@@ -187,6 +190,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t *__restr
         if (valid && (peers & lt) == 0) cnt[wave][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
+    uint32_t total;  // keys of digit t in the tile
     {  // exclusive scan over the 4 waves for digit t
         uint32_t s = 0;
 #pragma unroll
@@ -195,16 +199,47 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter(const uint32_t *__restr
             cnt[w][t] = s;
             s += x;
         }
+        total = s;
+    }
+    // exclusive scan of the digit totals across the tile: lbase
+    {
+        uint32_t x = total;  // inclusive scan inside each wave, then wave sums
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) lbase[wave] = x;  // wave sums, read back below
+        __syncthreads();
+        uint32_t wsum = 0;
+        for (int w = 0; w < wave; ++w) wsum += lbase[w];
+        __syncthreads();
+        lbase[t] = wsum + x - total;
     }
     __syncthreads();
+    // Keys into LDS in output order: digit-major, stable within a digit.
 #pragma unroll
     for (int c = 0; c < kRsChunks; ++c) {
-        const int64_t idx = tile0 + c * 64 + lane;
+        const int64_t idx = wave0 + c * 64 + lane;
         if (idx < n) {
             const uint32_t d = (k[c] >> shift) & 0xffu;
-            const uint32_t pos = base[d] + cnt[wave][d] + off[c];
-            kout[pos] = k[c];
-            vout[pos] = v[c];
+            const uint32_t l = lbase[d] + cnt[wave][d] + off[c];
+            sk[l] = k[c];
+            sv[l] = v[c];
+        }
+    }
+    __syncthreads();
+    // Consecutive lanes write consecutive slots of one digit's run.
+    const int valid_n = (int)min((int64_t)kRsTile, n - tile0);
+#pragma unroll
+    for (int c = 0; c < kRsChunks; ++c) {
+        const int l = c * kRsThreads + t;
+        if (l < valid_n) {
+            const uint32_t kk = sk[l];
+            const uint32_t d = (kk >> shift) & 0xffu;
+            const uint32_t pos = base[d] + (uint32_t)l - lbase[d];
+            kout[pos] = kk;
+            vout[pos] = sv[l];
         }
     }
 }
