@@ -408,12 +408,12 @@ namespace {
 
 void free_ws(msbg_learned *n) {
     for (uint16_t **p : {&n->A1, &n->A2, &n->A3, &n->G1, &n->G2, &n->G3, &n->G4, &n->X0}) {
-        hipFree(*p);
+        (void)hipFree(*p);
         *p = nullptr;
     }
-    hipFree(n->Y);
-    hipFree(n->wpart);
-    hipFree(n->loss_part);
+    (void)hipFree(n->Y);
+    (void)hipFree(n->wpart);
+    (void)hipFree(n->loss_part);
     n->Y = nullptr;
     n->wpart = nullptr;
     n->loss_part = nullptr;
@@ -598,13 +598,13 @@ int msbg_learned_new(msbg_learned **out) {
 int msbg_learned_free(msbg_learned *n) {
     if (!n) return MSBG_OK;
     free_ws(n);
-    hipFree(n->d_params);
-    hipFree(n->d_m);
-    hipFree(n->d_v);
-    hipFree(n->d_grad);
-    hipFree(n->d_w);
-    hipFree(n->d_wf);
-    hipFree(n->d_loss);
+    (void)hipFree(n->d_params);
+    (void)hipFree(n->d_m);
+    (void)hipFree(n->d_v);
+    (void)hipFree(n->d_grad);
+    (void)hipFree(n->d_w);
+    (void)hipFree(n->d_wf);
+    (void)hipFree(n->d_loss);
     delete n;
     return MSBG_OK;
 }
@@ -696,8 +696,8 @@ int msbg_learned_train(msbg_learned *n, const uint16_t *d_x0, const float *d_tar
     }
     if (e == hipSuccess) e = hipMemcpyAsync(host_losses, d_losses, (size_t)steps * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(d_idx);
-    hipFree(d_losses);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_losses);
     MSBG_HIP(e);
     return MSBG_OK;
 }

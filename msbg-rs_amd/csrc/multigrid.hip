@@ -803,18 +803,18 @@ void release(msbg_poisson *p) {
     if (!p || !p->mg) return;
     Hierarchy *H = p->mg;
     for (Level &L : H->lv) {
-        hipFree(L.x);
-        hipFree(L.b);
-        hipFree(L.nb);
-        hipFree(L.cf);
-        hipFree(L.diag);
-        hipFree(L.col);
-        hipFree(L.ch);
-        hipFree(L.pn);
-        hipFree(L.comp);
-        hipFree(L.sing);
+        (void)hipFree(L.x);
+        (void)hipFree(L.b);
+        (void)hipFree(L.nb);
+        (void)hipFree(L.cf);
+        (void)hipFree(L.diag);
+        (void)hipFree(L.col);
+        (void)hipFree(L.ch);
+        (void)hipFree(L.pn);
+        (void)hipFree(L.comp);
+        (void)hipFree(L.sing);
     }
-    hipFree(H->nb27);
+    (void)hipFree(H->nb27);
     delete H;
     p->mg = nullptr;
 }

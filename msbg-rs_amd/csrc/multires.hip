@@ -137,7 +137,7 @@ int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t 
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
-    hipFree(d_bids);
+    (void)hipFree(d_bids);
     MSBG_HIP(e);
     return MSBG_OK;
 }
@@ -158,7 +158,7 @@ int msbg_regularize_levels(uint8_t *dev_levels, int64_t nx, int64_t ny, int64_t 
         if (e == hipSuccess) e = hipMemcpyAsync(prev, dev_levels, nb, hipMemcpyDeviceToDevice, stream());
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
-    hipFree(prev);
+    (void)hipFree(prev);
     MSBG_HIP(e);
     return MSBG_OK;
 }

@@ -16,8 +16,8 @@ void *workspace(int id, size_t bytes, int *rc) {
     *rc = MSBG_OK;
     if (bytes <= g_ws_size[id]) return g_ws[id];
     if (g_ws[id]) {
-        hipStreamSynchronize(stream());
-        hipFree(g_ws[id]);
+        (void)hipStreamSynchronize(stream());
+        (void)hipFree(g_ws[id]);
         g_ws[id] = nullptr;
         g_ws_size[id] = 0;
     }

@@ -204,8 +204,8 @@ int msbg_grid_new(int64_t sx, int64_t sy, int64_t sz, int log2_bsx, float empty_
 
 int msbg_grid_free(msbg_grid *g) {
     if (!g) return MSBG_OK;
-    if (g->d_slab) hipFree(g->d_slab);
-    if (g->d_b2s) hipFree(g->d_b2s);
+    if (g->d_slab) (void)hipFree(g->d_slab);
+    if (g->d_b2s) (void)hipFree(g->d_b2s);
     delete g;
     return MSBG_OK;
 }
@@ -268,7 +268,7 @@ int msbg_grid_ensure_blocks(msbg_grid *g, const int64_t *bids, int64_t n) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
-    hipFree(d_slots);
+    (void)hipFree(d_slots);
     MSBG_HIP(e);
     return upload_b2s(g);
 }
@@ -290,7 +290,7 @@ int msbg_grid_fill_blocks(msbg_grid *g, const int64_t *bids, int64_t n, float va
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
-    hipFree(d_slots);
+    (void)hipFree(d_slots);
     MSBG_HIP(e);
     return MSBG_OK;
 }

@@ -464,7 +464,7 @@ using pband::read_norms;
 using pband::residual;
 
 void drop_graph(msbg_poisson *p) {
-    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
     p->exec = nullptr;
     p->g_mode = -1;
 }
@@ -557,8 +557,8 @@ int ensure_inner(msbg_poisson *p, int storage) {
     if (!p->d_r) MSBG_HIP(hipMalloc(&p->d_r, n * 4));
     if (p->inner_bytes < eb) {
         drop_graph(p);
-        if (p->d_e) hipFree(p->d_e);
-        if (p->d_rt) hipFree(p->d_rt);
+        if (p->d_e) (void)hipFree(p->d_e);
+        if (p->d_rt) (void)hipFree(p->d_rt);
         p->d_e = p->d_rt = nullptr;
         p->inner_bytes = 0;
         MSBG_HIP(hipMalloc(&p->d_e, n * eb));
@@ -618,7 +618,7 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
         hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
         MSBG_HIP(e);
         e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         MSBG_HIP(e);
         p->g_mode = mode;
         p->g_storage = storage;
@@ -747,21 +747,21 @@ int msbg_poisson_free(msbg_poisson *p) {
     if (!p) return MSBG_OK;
     drop_graph(p);
     mg::release(p);
-    if (p->jexec) hipGraphExecDestroy(p->jexec);
-    if (p->cap_stream) hipStreamDestroy(p->cap_stream);
-    hipFree(p->d_blocks);
-    hipFree(p->d_partial);
-    hipFree(p->d_norm);
-    hipFree(p->d_scale);
-    if (p->h_norm) hipHostFree(p->h_norm);
-    hipFree(p->d_r);
-    hipFree(p->d_e);
-    hipFree(p->d_rt);
-    hipFree(p->d_b2s);
-    hipFree(p->d_listed);
-    hipFree(p->d_delta);
-    hipFree(p->d_xtry);
-    hipFree(p->d_dots);
+    if (p->jexec) (void)hipGraphExecDestroy(p->jexec);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    (void)hipFree(p->d_blocks);
+    (void)hipFree(p->d_partial);
+    (void)hipFree(p->d_norm);
+    (void)hipFree(p->d_scale);
+    if (p->h_norm) (void)hipHostFree(p->h_norm);
+    (void)hipFree(p->d_r);
+    (void)hipFree(p->d_e);
+    (void)hipFree(p->d_rt);
+    (void)hipFree(p->d_b2s);
+    (void)hipFree(p->d_listed);
+    (void)hipFree(p->d_delta);
+    (void)hipFree(p->d_xtry);
+    (void)hipFree(p->d_dots);
     delete p;
     return MSBG_OK;
 }
@@ -795,7 +795,7 @@ int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp,
     if (!p->n_active || !sweeps) return MSBG_OK;
     if (!(p->jexec && p->j_storage == storage && p->j_sweeps == sweeps && p->j_omega == omega && p->j_x == x &&
           p->j_t == tmp && p->j_b == b)) {
-        if (p->jexec) hipGraphExecDestroy(p->jexec);
+        if (p->jexec) (void)hipGraphExecDestroy(p->jexec);
         p->jexec = nullptr;
         if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
@@ -806,7 +806,7 @@ int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp,
         hipGraph_t graph = nullptr;
         MSBG_HIP(hipStreamEndCapture(p->cap_stream, &graph));
         hipError_t e = hipGraphInstantiate(&p->jexec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         MSBG_HIP(e);
         p->j_storage = storage;
         p->j_sweeps = sweeps;
@@ -961,8 +961,8 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     MSBG_HIP(e);
     st->ms = ms;

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "primitives.hpp"
 
 namespace msbg {
 
@@ -596,7 +597,7 @@ int launch_iterations(msbg_sweep_plan *p, hipStream_t s, int stencil, float dt, 
 }
 
 void drop_graph(msbg_sweep_plan *p) {
-    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
     p->exec = nullptr;
     p->g_stencil = -1;
 }
@@ -614,8 +615,10 @@ int gather(const msbg_grid *g, const msbg_grid *coarse, const uint8_t *d_levels,
     for (int64_t i = 0; i < n; ++i)
         MSBG_CHECK(bids[i] >= 0 && bids[i] < g->n_blocks, MSBG_EINVAL, "halo gather: bid out of range");
     if (n == 0) return MSBG_OK;
-    int64_t *d_bids = nullptr;
-    MSBG_HIP(hipMalloc(&d_bids, sizeof(int64_t) * n));
+    int rc = MSBG_OK;
+    // the block list goes to a cached device buffer (no allocation per call)
+    int64_t *d_bids = (int64_t *)workspace(21, sizeof(int64_t) * n, &rc);
+    if (rc) return rc;
     hipError_t e = hipMemcpyAsync(d_bids, bids, sizeof(int64_t) * n, hipMemcpyHostToDevice, stream());
     if (e == hipSuccess) {
         GridView gv = g->view();
@@ -640,7 +643,6 @@ int gather(const msbg_grid *g, const msbg_grid *coarse, const uint8_t *d_levels,
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
-    hipFree(d_bids);
     MSBG_HIP(e);
     return MSBG_OK;
 }
@@ -691,10 +693,10 @@ int msbg_sweep_plan_new_multires(msbg_grid *fine, const msbg_grid *coarse, const
 int msbg_sweep_plan_free(msbg_sweep_plan *p) {
     if (!p) return MSBG_OK;
     drop_graph(p);
-    if (p->cap_stream) hipStreamDestroy(p->cap_stream);
-    if (p->d_bids) hipFree(p->d_bids);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->d_bids) (void)hipFree(p->d_bids);
     for (int32_t *t : p->d_topo)
-        if (t) hipFree(t);
+        if (t) (void)hipFree(t);
     delete p;
     return MSBG_OK;
 }
@@ -721,12 +723,12 @@ int msbg_sweep_plan_run_graph_async(msbg_sweep_plan *p, int stencil, float dt, i
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
         if (rc) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         MSBG_HIP(e);
         e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         MSBG_HIP(e);
         p->g_stencil = stencil;
         p->g_iters = iterations;
@@ -752,7 +754,7 @@ int msbg_sweep_plan_run_timed(msbg_sweep_plan *p, int stencil, float dt, int ite
         e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
         total += ms;
     }
-    for (hipEvent_t x : ev) hipEventDestroy(x);
+    for (hipEvent_t x : ev) (void)hipEventDestroy(x);
     if (rc) return rc;
     MSBG_HIP(e);
     *kernel_ms = total;
@@ -825,8 +827,8 @@ int msbg_halo_topology(const msbg_grid *g, const msbg_grid *coarse, const uint8_
     if (e == hipSuccess && rc == MSBG_OK)
         e = hipMemcpyAsync(host_out, d_topo, sizeof(int32_t) * nb * n, hipMemcpyDeviceToHost, stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
-    hipFree(d_bids);
-    if (d_topo) hipFree(d_topo);
+    (void)hipFree(d_bids);
+    if (d_topo) (void)hipFree(d_topo);
     if (rc) return rc;
     MSBG_HIP(e);
     return MSBG_OK;

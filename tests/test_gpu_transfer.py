@@ -162,6 +162,26 @@ def test_bspline_p2g_g2p_bitexact(pkg, orc, hip, order, n, s):
 
 
 @pytest.mark.parametrize("order", [1, 2])
+def test_bspline_p2g_dense_plane_bitexact(pkg, orc, hip, order):
+    """A voxel plane of a block's neighbourhood holding more particles than the
+    P2G stages in LDS (1,024) is read from memory directly; the planes around
+    it are staged.  Both give the oracle's bits."""
+    s = (48, 48, 48)
+    rng = np.random.default_rng(31 + order)
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(6000, 3)).astype(np.float32)
+    pos[:2500, :2] = rng.uniform(20.0, 23.0, size=(2500, 2))  # a 3 x 3 voxel patch ...
+    pos[:2500, 2] = rng.uniform(17.0, 18.0, size=2500)  # ... in one z plane
+    vel = rng.normal(size=(6000, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, 6000).astype(np.float32)
+    active, b2s, slabs = oracle_p2g_bspline(orc, order, s, pos, vel, mass)
+    tg = T.TransferGrid(*s, capacity_blocks=active.size + 4, lib=hip)
+    assert tg.p2g(order, pos, vel, mass) == active.size
+    assert_bitexact(tg.mass.to_host().to_dense(), dense_from_slabs(b2s, slabs[0], s), "mass")
+    for a in range(3):
+        assert_bitexact(tg.velocity[a].to_host().to_dense(), dense_from_slabs(b2s, slabs[1 + a], s), "vel %d" % a)
+
+
+@pytest.mark.parametrize("order", [1, 2])
 def test_bspline_g2p_staged_and_direct_chunks(pkg, orc, hip, order):
     """G2P chunks of 256 sorted particles whose support box fits the LDS
     window (a dense cluster) and chunks whose box does not (sparse particles
