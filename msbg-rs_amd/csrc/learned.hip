@@ -56,48 +56,75 @@ struct ConvArgs {
 
 // MFMA 16x16x16 bf16 lane layout: A[m][k] from lane m + 16 (k / 4), B[k][n]
 // from lane n + 16 (k / 4), D[m][n] in lane n + 16 (m / 4), element m % 4.
+//
+// A wave computes kConvGroups groups of 16 output cells x 16 channels: per
+// tap it loads the tap's weights once and issues one MFMA per group into that
+// group's own accumulator, so the groups' 27-step MFMA chains interleave.
+// Each group sees the same MFMA sequence as alone, so its bits do not depend
+// on kConvGroups.
+constexpr int kConvGroups = 4;
+
 template <int CIN, bool OUTF>
 __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
+    constexpr int G = kConvGroups;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int tile = wave / a.groups, grp = wave - tile * a.groups;
+    const int gpw = (a.groups + G - 1) / G;  // wave slots per tile
+    const int tile = wave / gpw, grp0 = (wave - tile * gpw) * G;
     if (tile >= a.n_tiles) return;  // wave-uniform
     const int ncell = a.sout * a.sout * a.sout;
     const int m = lane & 15, kg = lane >> 4;
-    int q = grp * 16 + m;
-    if (q >= ncell) q = ncell - 1;
-    const int qx = q % a.sout, qy = (q / a.sout) % a.sout, qz = q / (a.sout * a.sout);
+    int qx[G], qy[G], qz[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        int q = (grp0 + g) * 16 + m;
+        if (q >= ncell) q = ncell - 1;  // past the last cell: computed, not stored
+        qx[g] = q % a.sout, qy[g] = (q / a.sout) % a.sout, qz[g] = q / (a.sout * a.sout);
+    }
     const int64_t in_base = (int64_t)(a.idx ? a.idx[tile] : tile) * a.sin * a.sin * a.sin;
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    f4 acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = f4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int t = 0; t < 27; ++t) {
         const uint16_t *wp = a.w + (t * 16 + 4 * kg) * 16 + m;
         const s4 bv = {(short)wp[0], (short)wp[16], (short)wp[32], (short)wp[48]};
-        const int ix = qx + a.off + t % 3, iy = qy + a.off + (t / 3) % 3, iz = qz + a.off + t / 9;
-        s4 av = {0, 0, 0, 0};
-        if ((unsigned)ix < (unsigned)a.sin && (unsigned)iy < (unsigned)a.sin && (unsigned)iz < (unsigned)a.sin) {
-            const int64_t cell = in_base + ((int64_t)iz * a.sin + iy) * a.sin + ix;
-            if (CIN == 16) {
-                const uint2 v = *(const uint2 *)(a.in + cell * 16 + 4 * kg);
-                av = s4{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
-            } else if (kg == 0) {
-                const uint32_t v = *(const uint32_t *)(a.in + cell * 2);
-                av = s4{(short)(v & 0xffff), (short)(v >> 16), 0, 0};
+        const int ox = a.off + t % 3, oy = a.off + (t / 3) % 3, oz = a.off + t / 9;
+        s4 av[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int ix = qx[g] + ox, iy = qy[g] + oy, iz = qz[g] + oz;
+            av[g] = s4{0, 0, 0, 0};
+            if ((unsigned)ix < (unsigned)a.sin && (unsigned)iy < (unsigned)a.sin && (unsigned)iz < (unsigned)a.sin) {
+                const int64_t cell = in_base + ((int64_t)iz * a.sin + iy) * a.sin + ix;
+                if (CIN == 16) {
+                    const uint2 v = *(const uint2 *)(a.in + cell * 16 + 4 * kg);
+                    av[g] = s4{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
+                } else if (kg == 0) {
+                    const uint32_t v = *(const uint32_t *)(a.in + cell * 2);
+                    av[g] = s4{(short)(v & 0xffff), (short)(v >> 16), 0, 0};
+                }
             }
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[g], bv, acc[g], 0, 0, 0);
     }
     const float bias = a.bias ? a.bias[m] : 0.0f;
-    for (int j = 0; j < 4; ++j) {
-        const int cell = grp * 16 + 4 * kg + j;
-        if (cell >= ncell) continue;
-        float v = acc[j] + bias;
-        if (a.relu) v = v > 0.0f ? v : 0.0f;
-        const int64_t o = (int64_t)tile * ncell + cell;
-        if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
-        if (OUTF) {
-            if (m == 0) ((float *)a.out)[o] = v;
-        } else {
-            ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int grp = grp0 + g;
+        if (grp >= a.groups) continue;
+        for (int j = 0; j < 4; ++j) {
+            const int cell = grp * 16 + 4 * kg + j;
+            if (cell >= ncell) continue;
+            float v = acc[g][j] + bias;
+            if (a.relu) v = v > 0.0f ? v : 0.0f;
+            const int64_t o = (int64_t)tile * ncell + cell;
+            if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
+            if (OUTF) {
+                if (m == 0) ((float *)a.out)[o] = v;
+            } else {
+                ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
+            }
         }
     }
 }
@@ -459,7 +486,7 @@ void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int so
     a.out = out;
     a.groups = (sout * sout * sout + 15) / 16;
     a.n_tiles = n;
-    const int64_t waves = (int64_t)a.groups * n;
+    const int64_t waves = (int64_t)((a.groups + kConvGroups - 1) / kConvGroups) * n;
     conv_kernel<CIN, OUTF><<<(int)((waves + 3) / 4), 256, 0, s>>>(a);
 }
 
