@@ -167,10 +167,10 @@ void orc_halo_fill(i64 sx, i64 sy, i64 sz, int L, const float *slab, const i32 *
         okr[j] = resolve_axis(x0 + B + j, (i32)sx - 1, bc, &rxr[j]);
     }
     for (int z = 0; z < HSX; ++z) {
-        i32 rz;
+        i32 rz = 0;
         bool okz = resolve_axis(bz0 * B + z - halo, (i32)sz - 1, bc, &rz);
         for (int y = 0; y < HSX; ++y) {
-            i32 ry;
+            i32 ry = 0;
             bool oky = resolve_axis(by0 * B + y - halo, (i32)sy - 1, bc, &ry);
             float *row = tile + (size_t)z * HSX * HSX + (size_t)y * HSX;
             if (oky && okz && x0 + B <= sx) {  // halo.rs:117-129 contiguous middle row
@@ -251,10 +251,10 @@ void orc_halo_fill_multires(i64 sx, i64 sy, i64 sz, int L, const float *fine_sla
         okr[j] = resolve_axis(x0 + B + j, (i32)sx - 1, bc, &rxr[j]);
     }
     for (int z = 0; z < HSX; ++z) {
-        i32 rz;
+        i32 rz = 0;
         bool okz = resolve_axis(bz0 * B + z - halo, (i32)sz - 1, bc, &rz);
         for (int y = 0; y < HSX; ++y) {
-            i32 ry;
+            i32 ry = 0;
             bool oky = resolve_axis(by0 * B + y - halo, (i32)sy - 1, bc, &ry);
             float *row = tile + (size_t)z * HSX * HSX + (size_t)y * HSX;
             if (oky && okz && x0 + B <= sx) {

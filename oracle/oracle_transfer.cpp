@@ -472,7 +472,7 @@ uint16_t quant_dsq(float d2, float inv) {
 
 extern "C" void orc_density_splat(i64 sx, i64 sy, i64 sz, const i32 *b2s, uint16_t *slab, const float *pos, i64 n,
                                   float r_scan, int ref_commit_width) {
-    const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nz = (sz + 15) >> 4, nxy = nx * ny;
+    const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nxy = nx * ny;
     const int m = (int)ceilf(r_scan);
     const int W = 16 + 2 * m;
     const int MSX = std::max(24, (W + 7) & ~7);  // staging row pitch (the reference's MSX, a multiple of 8)
