@@ -126,8 +126,11 @@ typedef struct msbg_sweep_plan msbg_sweep_plan;
 int msbg_sweep_plan_new(msbg_grid *g, const int64_t *active, int64_t n, msbg_sweep_plan **out);
 int msbg_sweep_plan_free(msbg_sweep_plan *p);
 int msbg_sweep_plan_run_async(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01);
-/* The same passes replayed as one HIP graph (captured on first use for a
- * given stencil / dt / iterations / clamp01 and cached in the plan). */
+/* The same passes replayed as one HIP graph, captured on first use of a
+ * stencil / dt / iterations / clamp01 combination.  The plan caches the
+ * graphs of its 4 most recently used combinations, so alternating stencils
+ * replay without re-capture; an evicted graph is destroyed only after its
+ * last launch has completed. */
 int msbg_sweep_plan_run_graph_async(msbg_sweep_plan *p, int stencil, float dt, int iterations, int clamp01);
 /* The same passes with a HIP event pair around every colour launch: returns
  * the summed kernel time and the launch count (synchronizes). */
@@ -146,6 +149,27 @@ int msbg_sweep_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *act
 /* A plan over a u16 slab (run with msbg_sweep_plan_run_async / _run_graph_async). */
 int msbg_sweep_plan_new_density(msbg_grid *layout, uint16_t *d_slab16, const int64_t *active, int64_t n,
                                 msbg_sweep_plan **out);
+
+/* Sweeper<Density8> -- solver.rs:154-240 with D = Density8 (u8,
+ * channel.rs:55-59,155-189): the same sweeps over a u8 slab d_slab8 laid out
+ * by `layout`'s bid -> slot table.  The halo is dequantised linearly as
+ * u * (1/255) (the Density8 Dequant, math/sample.rs:64-69: the sqrt-space
+ * value) and every result v is stored sqrt-compressed with stochastic
+ * rounding (StoreBack<Density8>, math/stencil.rs:176-192): vf = sqrt(v) * 255,
+ * code = floor(vf) + (u < vf - floor(vf)), saturated to [0, 255] (v is meant
+ * to lie in [0, 1]: clamp01 = 1).  The reference draws u from per-thread
+ * lane generators whose stream depends on its SIMD width and schedule; here u
+ * is a fixed function of (seed, sweep index, bid, voxel): h0 = fmix32(seed ^
+ * sweep * 0x9e3779b9), h = fmix32(fmix32(h0 ^ bid) ^ vid * 0x9e3779b9) (murmur3
+ * finaliser; 0 -> 0xdeadbeef), u = float((h * 16807) >> 9 as a [1, 2)
+ * mantissa) - 1.  The sweep index counts the sweeps of the call (or of the
+ * plan, which keeps counting across runs and graph replays) from 0.
+ * Single level. */
+int msbg_sweep_density8(msbg_grid *layout, uint8_t *d_slab8, const int64_t *active, int64_t n, int stencil, float dt,
+                        int iterations, int clamp01, uint32_t seed);
+/* A plan over a u8 slab; its sweep index starts at 0. */
+int msbg_sweep_plan_new_density8(msbg_grid *layout, uint8_t *d_slab8, const int64_t *active, int64_t n,
+                                 uint32_t seed, msbg_sweep_plan **out);
 
 /* Halo topology -- the neighbour resolution of halo.rs:59-80 (single level,
  * coarse = NULL) or halo.rs:234-251 (multires): for each bid a row of
@@ -176,6 +200,28 @@ int msbg_sweep_multires(msbg_grid *fine, const msbg_grid *coarse, const uint8_t 
                         const int64_t *active, int64_t n, int stencil, float dt, int iterations, int clamp01);
 int msbg_sweep_plan_new_multires(msbg_grid *fine, const msbg_grid *coarse, const uint8_t *dev_levels,
                                  const int64_t *active, int64_t n, msbg_sweep_plan **out);
+/* The 8-colour sweep of a multires grid's blocks at every level (extends
+ * fill_multires, multires/halo.rs:187-358, whose centre block is always fine,
+ * to centre blocks at coarse levels; Sweeper::sweep's colour schedule,
+ * solver.rs:166-267).  grids[l] (l < n_levels, 2 <= n_levels <= 5) is level
+ * l with block edge 16 >> l over one block grid; dev_levels[bid] is the
+ * block's level; each bid of `active` (host) is updated at its own level, in
+ * place, and skipped when it has no value block there.  A block's colour is
+ * the parity of its block coordinates at any level, and one colour pass
+ * updates the active blocks of that colour at all levels.  Halos: a level-0
+ * block's is msbg_halo_gather_multires's; a block at level l >= 1 reads each
+ * halo voxel g (Neumann mirror of level l's domain) from the level its block
+ * lives at: level l itself, level l+1 at g >> 1 (when coarser), or the mean
+ * of level l-1's 2^3 voxels 2g + {0,1}^3 summed z, y, x in order and divided
+ * by 8 (when finer; downsample_channel_avg's arithmetic,
+ * multires/downsample.rs:44-53).  The kernels are the reference's on the
+ * level's block edge.  Edge-1 blocks (level 4) take radius-1 stencils only. */
+int msbg_sweep_levels(msbg_grid *const *grids, int n_levels, const uint8_t *dev_levels, const int64_t *active,
+                      int64_t n, int stencil, float dt, int iterations, int clamp01);
+/* The same sweep as a reusable plan (run with msbg_sweep_plan_run_async /
+ * _run_graph_async / _run_timed). */
+int msbg_sweep_plan_new_levels(msbg_grid *const *grids, int n_levels, const uint8_t *dev_levels,
+                               const int64_t *active, int64_t n, msbg_sweep_plan **out);
 /* downsample_channel_avg -- multires/downsample.rs:13-58 for `bids` (host);
  * the coarse blocks are ensured first. */
 int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t *bids, int64_t n);

@@ -12,6 +12,7 @@ from .lib import MsbgError, DeviceArray  # noqa: F401
 from .device import (  # noqa: F401
     BoundaryCondition, DeviceGrid, PdeParams, Stencil, SweepPlan, Sweeper, apply_pde, color_buckets,
     downsample_channel_avg, halo_fill, halo_fill_multires, halo_topology, regularize_refinement_map, sweep_multires,
+    sweep_levels,
     compute_block_topology, Topology,
 )
 from .solve import PoissonBand, Storage  # noqa: F401

@@ -550,10 +550,21 @@ int correct(msbg_poisson *p, msbg_learned *net, const msbg_grid *phi, float *x, 
     hipStream_t s = stream();
     const size_t slab = (size_t)p->n_slots * 4096;
     if (!p->d_delta) {
-        MSBG_HIP(hipMalloc(&p->d_delta, slab * 4));
-        MSBG_HIP(hipMalloc(&p->d_xtry, slab * 4));
-        MSBG_HIP(hipMalloc(&p->d_dots, 2 * sizeof(double)));
-        fill_zero_kernel<<<1024, 256, 0, s>>>(p->d_delta, (int64_t)slab);
+        // all three buffers or none: a failed allocation leaves the band unchanged
+        float *delta = nullptr, *xtry = nullptr;
+        double *dots = nullptr;
+        hipError_t e = hipMalloc(&delta, slab * 4);
+        if (e == hipSuccess) e = hipMalloc(&xtry, slab * 4);
+        if (e == hipSuccess) e = hipMalloc(&dots, 2 * sizeof(double));
+        if (e != hipSuccess) {
+            if (delta) (void)hipFree(delta);
+            if (xtry) (void)hipFree(xtry);
+            return hip_fail(e, "learned correction buffers");
+        }
+        fill_zero_kernel<<<1024, 256, 0, s>>>(delta, (int64_t)slab);
+        p->d_delta = delta;
+        p->d_xtry = xtry;
+        p->d_dots = dots;
     }
     const int64_t chunk = std::min<int64_t>(p->n_active, 512);
     int rc = ensure_ws(net, chunk, false);

@@ -114,8 +114,11 @@ struct msbg_poisson {
     // learned correction scratch
     float *d_delta = nullptr, *d_xtry = nullptr;
     double *d_dots = nullptr;  // [2]: r . A delta, |A delta|^2
-    // graph of one check period, keyed by its arguments
+    // graph of one check period, keyed by its arguments; g_done / j_done are
+    // recorded after each launch, and an exec is destroyed only once its
+    // event has completed
     hipStream_t cap_stream = nullptr;
+    hipEvent_t g_done = nullptr, j_done = nullptr;
     hipGraphExec_t exec = nullptr;
     int g_mode = -1, g_storage = -1, g_sweeps = -1;
     float g_omega = 0.0f;

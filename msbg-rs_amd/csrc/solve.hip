@@ -464,9 +464,21 @@ using pband::read_norms;
 using pband::residual;
 
 void drop_graph(msbg_poisson *p) {
-    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->exec) {
+        if (p->g_done) (void)hipEventSynchronize(p->g_done);
+        (void)hipGraphExecDestroy(p->exec);
+    }
     p->exec = nullptr;
     p->g_mode = -1;
+}
+
+void drop_jacobi_graph(msbg_poisson *p) {
+    if (p->jexec) {
+        if (p->j_done) (void)hipEventSynchronize(p->j_done);
+        (void)hipGraphExecDestroy(p->jexec);
+    }
+    p->jexec = nullptr;
+    p->j_sweeps = -1;
 }
 
 int check_storage(int st) {
@@ -611,6 +623,7 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
     if (!(p->exec && p->g_mode == mode && p->g_storage == storage && p->g_sweeps == sweeps && p->g_omega == omega &&
           p->g_x == x && p->g_b == b)) {
         drop_graph(p);
+        if (!p->g_done) MSBG_HIP(hipEventCreateWithFlags(&p->g_done, hipEventDisableTiming));
         if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
         period(p, p->cap_stream, mode, storage, x, b, omega, sweeps);
@@ -619,6 +632,7 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
         MSBG_HIP(e);
         e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) p->exec = nullptr;
         MSBG_HIP(e);
         p->g_mode = mode;
         p->g_storage = storage;
@@ -628,6 +642,7 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
         p->g_b = b;
     }
     MSBG_HIP(hipGraphLaunch(p->exec, stream()));
+    MSBG_HIP(hipEventRecord(p->g_done, stream()));
     return MSBG_OK;
 }
 
@@ -746,8 +761,11 @@ int msbg_poisson_new(const msbg_grid *layout, const int64_t *active, int64_t n_a
 int msbg_poisson_free(msbg_poisson *p) {
     if (!p) return MSBG_OK;
     drop_graph(p);
+    drop_jacobi_graph(p);
+    (void)hipStreamSynchronize(stream());  // the band's buffers may still be in use by queued work
     mg::release(p);
-    if (p->jexec) (void)hipGraphExecDestroy(p->jexec);
+    if (p->g_done) (void)hipEventDestroy(p->g_done);
+    if (p->j_done) (void)hipEventDestroy(p->j_done);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     (void)hipFree(p->d_blocks);
     (void)hipFree(p->d_partial);
@@ -795,8 +813,8 @@ int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp,
     if (!p->n_active || !sweeps) return MSBG_OK;
     if (!(p->jexec && p->j_storage == storage && p->j_sweeps == sweeps && p->j_omega == omega && p->j_x == x &&
           p->j_t == tmp && p->j_b == b)) {
-        if (p->jexec) (void)hipGraphExecDestroy(p->jexec);
-        p->jexec = nullptr;
+        drop_jacobi_graph(p);
+        if (!p->j_done) MSBG_HIP(hipEventCreateWithFlags(&p->j_done, hipEventDisableTiming));
         if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
         for (int i = 0; i < sweeps; i += 2) {
@@ -807,6 +825,7 @@ int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp,
         MSBG_HIP(hipStreamEndCapture(p->cap_stream, &graph));
         hipError_t e = hipGraphInstantiate(&p->jexec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) p->jexec = nullptr;
         MSBG_HIP(e);
         p->j_storage = storage;
         p->j_sweeps = sweeps;
@@ -816,6 +835,7 @@ int msbg_poisson_jacobi_sweeps(msbg_poisson *p, int storage, void *x, void *tmp,
         p->j_b = b;
     }
     MSBG_HIP(hipGraphLaunch(p->jexec, stream()));
+    MSBG_HIP(hipEventRecord(p->j_done, stream()));
     return MSBG_OK;
 }
 

@@ -278,32 +278,59 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
     // channel k of this pass is the mass when CH is 0, or CH is 2 and k is 0
     auto is_mass = [](int k) { return CH == 2 ? k == 0 : CH == 0; };
     auto scale_of = [&](int pi, int k) { return is_mass(k) ? smass[pi] : smass[pi] * svel[3 * pi + a]; };
+    // A particle whose mass or mass * velocity is not finite stays out of the
+    // fixed-point window (its sum has no scale); it is added in float after
+    // the commit, so only its own footprint turns non-finite.
     float bsum[NC];
+    bool nonfin = false;
 #pragma unroll
     for (int k = 0; k < NC; ++k) bsum[k] = 0.0f;
     for (int j = 0; j < per; ++j) {
         const int pi = p_beg + (int)threadIdx.x * per + j;
         if (pi < p_beg + p_cnt)
 #pragma unroll
-            for (int k = 0; k < NC; ++k) bsum[k] += fabsf(scale_of(pi, k));
+            for (int k = 0; k < NC; ++k) {
+                const float v = scale_of(pi, k);
+                if (isfinite(v))
+                    bsum[k] += fabsf(v);
+                else
+                    nonfin = true;
+            }
     }
     // w <= 1; the FP32 sum of the bounds is within a factor 1 + 2^-12 of its
     // value, and e leaves a factor 2 spare.
     float up[NC];
     int ex[NC];
-    bool any = false, finite = true;
+    bool any = false;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         const float bound = splat_wg_sum(bsum[k], s_red);
-        finite = finite && isfinite(bound);
         any = any || bound > 0.0f;
         frexpf(bound > 0.0f && isfinite(bound) ? bound : 1.0f, &ex[k]);  // bound < 2^e
         up[k] = ldexpf(1.0f, 61 - ex[k]);                                 // 2^S
     }
-    // Nothing but zero contributions, or non-finite particle data (whose sums
-    // have no fixed-point scale): the item adds nothing.  Uniform per group.
-    if (!any || !finite) return;
+    const bool item_nonfin = splat_wg_sum(nonfin ? 1.0f : 0.0f, s_red) > 0.0f;  // uniform per group
+    // Nothing but zero contributions: the item adds nothing.  Uniform per group.
+    if (!any && !item_nonfin) return;
     constexpr float kExact = 16777216.0f;  // 2^24
+    auto footprint = [&](int pi, int &ix1, int &ix2, int &iy1, int &iy2, int &iz1, int &iz2) {
+        const float px = spos[3 * pi], py = spos[3 * pi + 1], pz = spos[3 * pi + 2];
+        ix1 = min(max((int)ceilf(px - r_p - ofx), b0x - m), b0x + whi);
+        ix2 = min(max((int)floorf(px + r_p - ofx), b0x - m), b0x + whi);
+        iy1 = min(max((int)ceilf(py - r_p - ofy), b0y - m), b0y + whi);
+        iy2 = min(max((int)floorf(py + r_p - ofy), b0y - m), b0y + whi);
+        iz1 = min(max((int)ceilf(pz - r_p - ofz), b0z - m), b0z + whi);
+        iz2 = min(max((int)floorf(pz + r_p - ofz), b0z - m), b0z + whi);
+    };
+    // Eq. 6 weight of face cell (ix, iy, iz) for the particle at p0 = p - 1/2.
+    auto weight = [&](float px0, float py0, float pz0, int ix, int iy, int iz) {
+        const float dz = (float)iz - pz0, dy = (float)iy - py0, dx = (float)ix - px0;
+        const float dist_sq = dx * dx + (dz * dz + dy * dy);
+        const float d_a = a == 0 ? dx : (a == 1 ? dy : dz);
+        const float df2 = dist_sq + d_a + 0.25f;
+        const float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
+        return w * w * w;
+    };
     for (int j = 0; j < per; ++j) {
         const int pi = p_beg + (int)threadIdx.x * per + j;
         if (pi >= p_beg + p_cnt) break;
@@ -312,12 +339,8 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
 #pragma unroll
         for (int k = 0; k < NC; ++k) sc[k] = scale_of(pi, k);
         const float px0 = px - 0.5f, py0 = py - 0.5f, pz0 = pz - 0.5f;
-        const int ix1 = min(max((int)ceilf(px - r_p - ofx), b0x - m), b0x + whi);
-        const int ix2 = min(max((int)floorf(px + r_p - ofx), b0x - m), b0x + whi);
-        const int iy1 = min(max((int)ceilf(py - r_p - ofy), b0y - m), b0y + whi);
-        const int iy2 = min(max((int)floorf(py + r_p - ofy), b0y - m), b0y + whi);
-        const int iz1 = min(max((int)ceilf(pz - r_p - ofz), b0z - m), b0z + whi);
-        const int iz2 = min(max((int)floorf(pz + r_p - ofz), b0z - m), b0z + whi);
+        int ix1, ix2, iy1, iy2, iz1, iz2;
+        footprint(pi, ix1, ix2, iy1, iy2, iz1, iz2);
         for (int iz = iz1; iz <= iz2; ++iz) {
             const float dz = (float)iz - pz0;
             const float dz2 = dz * dz;
@@ -336,7 +359,7 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
 #pragma unroll
                     for (int k = 0; k < NC; ++k) {
                         const float c = w * sc[k];
-                        if (c == 0.0f) continue;  // adds +-0: the window keeps its value
+                        if (c == 0.0f || !isfinite(sc[k])) continue;  // +-0 keeps the value; non-finite: below
                         const float q = c * up[k];  // exact: a power-of-two scale within range
                         if (fabsf(q) >= kExact) {
                             atomicAdd(&acc[k * NS + cell], (unsigned long long)(long long)q);
@@ -395,16 +418,10 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
         }
     }
     __syncthreads();
-    // Commit (p2g.rs:342-411): staging cell -> its region -> the region's block.
-    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
-        float val[NC];
-        bool nz = false;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            val[k] = fv[k][s];
-            nz = nz || val[k] != 0.0f;
-        }
-        if (!nz) continue;
+    // Commit (p2g.rs:342-411): staging cell -> its region -> the region's
+    // block.  The target of staging cell s for channel k, or null where the
+    // region is not committed.
+    auto target = [&](int s, int k) -> float * {
         const int lx = s % MSX, ly = (s / MSX) % MSX, lz = s / (MSX * MSX);
         const int dx = lx < m ? -1 : (lx < m + 16 ? 0 : 1);
         const int dy = ly < m ? -1 : (ly < m + 16 ? 0 : 1);
@@ -413,22 +430,60 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
         const int ylo = dy < 0 ? 0 : (dy == 0 ? m : m + 16), ylen = dy == 0 ? 16 : m;
         const int zlo = dz < 0 ? 0 : (dz == 0 ? m : m + 16), zlen = dz == 0 ? 16 : m;
         const int gx0 = b0x + xlo - m, gy0 = b0y + ylo - m, gz0 = b0z + zlo - m;
-        if (gx0 < 0 || gy0 < 0 || gz0 < 0) continue;
-        if (gx0 + xlen > gm.sx || gy0 + ylen > gm.sy || gz0 + zlen > gm.sz) continue;
+        if (gx0 < 0 || gy0 < 0 || gz0 < 0) return nullptr;
+        if (gx0 + xlen > gm.sx || gy0 + ylen > gm.sy || gz0 + zlen > gm.sz) return nullptr;
         const int64_t nb = (int64_t)(bx + dx) + (int64_t)(by + dy) * gm.nx + (int64_t)(bz + dz) * gm.nx * gm.ny;
         const int gx = x0s + lx, gy = y0s + ly, gz = z0s + lz;
         const int vid = (gx & 15) | ((gy & 15) << 4) | ((gz & 15) << 8);
         const int ms = gm.b2s[nb], vs = gv.b2s[nb];
-        if (ms < 2 || vs < 2) continue;
+        if (ms < 2 || vs < 2) return nullptr;
+        return is_mass(k) ? gm.slab + (int64_t)ms * 4096 + vid : gv.slab + (int64_t)vs * 4096 + vid;
+    };
+    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-            if (val[k] == 0.0f) continue;
-            float *po = is_mass(k) ? gm.slab + (int64_t)ms * 4096 + vid : gv.slab + (int64_t)vs * 4096 + vid;
+            const float val = fv[k][s];
+            if (val == 0.0f) continue;
+            float *po = target(s, k);
+            if (!po) continue;
             if (item.shared)  // blocks of one colour never share a window: only the block's own items race
-                unsafeAtomicAdd(po, val[k]);
+                unsafeAtomicAdd(po, val);
             else
-                *po += val[k];
+                *po += val;
         }
+    }
+    if (!item_nonfin) return;
+    // Particles with a non-finite mass or momentum scale: their float
+    // contributions go straight to the committed cells (NaN / Inf propagate
+    // as in a float sum; finite cells outside their footprints keep the exact
+    // result above).
+    __syncthreads();
+    for (int j = 0; j < per; ++j) {
+        const int pi = p_beg + (int)threadIdx.x * per + j;
+        if (pi >= p_beg + p_cnt) break;
+        float sc[NC];
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            sc[k] = scale_of(pi, k);
+            bad = bad || !isfinite(sc[k]);
+        }
+        if (!bad) continue;
+        const float px0 = spos[3 * pi] - 0.5f, py0 = spos[3 * pi + 1] - 0.5f, pz0 = spos[3 * pi + 2] - 0.5f;
+        int ix1, ix2, iy1, iy2, iz1, iz2;
+        footprint(pi, ix1, ix2, iy1, iy2, iz1, iz2);
+        for (int iz = iz1; iz <= iz2; ++iz)
+            for (int iy = iy1; iy <= iy2; ++iy)
+                for (int ix = ix1; ix <= ix2; ++ix) {
+                    const float w = weight(px0, py0, pz0, ix, iy, iz);
+                    const int cell = ((iz - z0s) * MSX + (iy - y0s)) * MSX + ix - x0s;
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) {
+                        if (isfinite(sc[k])) continue;
+                        float *po = target(cell, k);
+                        if (po) atomicAdd(po, w * sc[k]);
+                    }
+                }
     }
 }
 

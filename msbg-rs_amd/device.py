@@ -186,13 +186,25 @@ def apply_pde(grid: DeviceGrid, active, stencil: Stencil, params: PdeParams, num
 
 
 class SweepPlan:
-    """Colour buckets resident on the device, re-run without host work."""
+    """Colour buckets resident on the device, re-run without host work.
 
-    def __init__(self, grid: DeviceGrid, active, coarse: DeviceGrid = None, levels: DeviceArray = None):
+    Single level (`coarse` None), fine blocks with cross-level halos
+    (`coarse` = level 1, `levels` = the device level map), or every level of a
+    multires grid (`level_grids` = [level 0, level 1, ...], `levels`): the
+    active blocks are then swept at their own levels.
+    """
+
+    def __init__(self, grid: DeviceGrid, active, coarse: DeviceGrid = None, levels: DeviceArray = None,
+                 level_grids=None):
         self.grid, self.coarse, self.levels = grid, coarse, levels
         a = _i64(active)
         h = C.c_void_p()
-        if coarse is None:
+        if level_grids is not None:
+            self._grids = list(level_grids)
+            arr = (C.c_void_p * len(self._grids))(*[x.h for x in self._grids])
+            grid.lib.call("msbg_sweep_plan_new_levels", arr, len(self._grids), levels.ptr, ptr(a), a.size,
+                          C.byref(h))
+        elif coarse is None:
             grid.lib.call("msbg_sweep_plan_new", grid.h, ptr(a), a.size, C.byref(h))
         else:
             grid.lib.call("msbg_sweep_plan_new_multires", grid.h, coarse.h, levels.ptr, ptr(a), a.size, C.byref(h))
@@ -258,6 +270,16 @@ def sweep_multires(fine: DeviceGrid, coarse: DeviceGrid, levels: DeviceArray, ac
     a = _i64(active)
     fine.lib.call("msbg_sweep_multires", fine.h, coarse.h, levels.ptr, ptr(a), a.size, int(stencil), params.dt,
                   params.iterations, int(bool(params.do_constr_zero_one)))
+
+
+def sweep_levels(level_grids, levels: DeviceArray, active, stencil, params: PdeParams):
+    """The 8-colour sweep of every active block at its own level (level l of
+    `level_grids` has block edge 16 >> l); coarse-level halos read finer
+    neighbours as 2^3 means and coarser ones by injection."""
+    a = _i64(active)
+    arr = (C.c_void_p * len(level_grids))(*[x.h for x in level_grids])
+    level_grids[0].lib.call("msbg_sweep_levels", arr, len(level_grids), levels.ptr, ptr(a), a.size, int(stencil),
+                            params.dt, params.iterations, int(bool(params.do_constr_zero_one)))
 
 
 def downsample_channel_avg(coarse: DeviceGrid, fine: DeviceGrid, bids):

@@ -54,6 +54,8 @@ _SIGS = {
     "msbg_sweep_plan_free": [vp],
     "msbg_sweep_density": [vp, vp, vp, i64, i32, f32, i32, i32],
     "msbg_sweep_plan_new_density": [vp, vp, vp, i64, PP],
+    "msbg_sweep_density8": [vp, vp, vp, i64, i32, f32, i32, i32, C.c_uint32],
+    "msbg_sweep_plan_new_density8": [vp, vp, vp, i64, C.c_uint32, PP],
     "msbg_sweep_plan_run_async": [vp, i32, f32, i32, i32],
     "msbg_sweep_plan_run_timed": [vp, i32, f32, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_int64)],
     "msbg_sweep_plan_run_graph_async": [vp, i32, f32, i32, i32],
@@ -62,6 +64,8 @@ _SIGS = {
     "msbg_halo_gather_multires": [vp, vp, vp, vp, i64, i32, i32, i32, vp],
     "msbg_sweep_multires": [vp, vp, vp, vp, i64, i32, f32, i32, i32],
     "msbg_sweep_plan_new_multires": [vp, vp, vp, vp, i64, PP],
+    "msbg_sweep_levels": [vp, i32, vp, vp, i64, i32, f32, i32, i32],
+    "msbg_sweep_plan_new_levels": [vp, i32, vp, vp, i64, PP],
     "msbg_downsample_avg": [vp, vp, vp, i64],
     "msbg_regularize_levels": [vp, i64, i64, i64, i32],
     "msbg_morton_keys": [vp, i64, vp],

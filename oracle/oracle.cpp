@@ -502,6 +502,83 @@ void orc_sweep_density(i64 sx, i64 sy, i64 sz, uint16_t *slab16, i64 n_slots, co
     }
 }
 
+// Sweeper::sweep for D = Density8 (u8, channel.rs:55-59,155-189): the halo
+// is filled from the linear dequantisation u * (1/255) (the Density8 Dequant,
+// math/sample.rs:64-69) and every smoothed value v is stored sqrt-compressed
+// with stochastic rounding (StoreBack<Density8>, math/stencil.rs:176-192):
+// vf = sqrt(v) * 255, code = floor(vf) + (u < vf - floor(vf)), saturated to
+// [0, 255].  The reference's draw u comes from per-thread SimdRng lanes
+// (math/stencil.rs:101-132), a stream that depends on the SIMD width and the
+// schedule; this restatement uses the stream the C ABI states for it, a fixed
+// function of (seed, sweep, bid, voxel) built from SimdRng's own pieces (the
+// murmur3 finaliser of SimdRng::seed and the x16807 step of SimdRng::next).
+// Sweep k of the call is sweep index sweep0 + k.
+static inline u32 d8_fmix(u32 h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+float orc_density8_uniform(u32 seed, u32 sweep, u32 bid, u32 vid) {
+    const u32 key = d8_fmix(seed ^ (sweep * 0x9e3779b9u));
+    u32 h = d8_fmix(d8_fmix(key ^ bid) ^ (vid * 0x9e3779b9u));
+    if (h == 0u) h = 0xdeadbeefu;
+    const u32 bits = ((h * 16807u) >> 9) | 0x3f800000u;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f - 1.0f;
+}
+
+uint8_t orc_density8_code(float v, float u) {
+    const float vf = sqrtf(v) * 255.0f;
+    const float fl = floorf(vf);
+    const float r = u < vf - fl ? fl + 1.0f : fl;
+    if (r >= 255.0f) return 255;
+    return r > 0.0f ? (uint8_t)(int)r : 0;
+}
+
+void orc_sweep_density8(i64 sx, i64 sy, i64 sz, uint8_t *slab8, i64 n_slots, const i32 *b2s, const i64 *active,
+                        i64 n, int stencil, float dt, int iterations, int clamp01, u32 seed, u32 sweep0) {
+    const int L = 4, B = 16;
+    const float inv = 1.0f / 255.0f;
+    std::vector<float> f((size_t)n_slots * 4096);
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)slab8[i] * inv;
+    std::vector<i64> order(n > 0 ? n : 1);
+    i64 counts[8];
+    orc_color_buckets(sx, sy, sz, L, active, n, order.data(), counts);
+    const int halo = stencil == 2 ? 2 : 1;
+    const int full = stencil == 0 ? 0 : 1;
+    const int HSX = B + 2 * halo;
+    std::vector<float> tile((size_t)HSX * HSX * HSX), out(4096);
+    for (int it = 0; it < iterations; ++it) {
+        i64 off = 0;
+        for (int c = 0; c < 8; ++c) {
+            for (i64 k = 0; k < counts[c]; ++k) {
+                const i64 bid = order[off + k];
+                const i32 s = b2s[bid];
+                if (s < 2) continue;
+                orc_halo_fill(sx, sy, sz, L, f.data(), b2s, bid, halo, full, BC_NEUMANN, tile.data());
+                if (stencil == 0)
+                    orc_kernel_laplacian(B, tile.data(), dt, clamp01, out.data());
+                else if (stencil == 1)
+                    orc_kernel_meancurv(B, tile.data(), dt, clamp01, out.data());
+                else
+                    orc_kernel_bilaplacian(B, tile.data(), dt, clamp01, out.data());
+                for (int v = 0; v < 4096; ++v) {
+                    const float u = orc_density8_uniform(seed, sweep0 + (u32)it, (u32)bid, (u32)v);
+                    const uint8_t q = orc_density8_code(out[v], u);
+                    slab8[(size_t)s * 4096 + v] = q;
+                    f[(size_t)s * 4096 + v] = (float)q * inv;
+                }
+            }
+            off += counts[c];
+        }
+    }
+}
+
 // Multires sweep: the same colour schedule over the level-l blocks in
 // `active`, with the halo gathered by fill_multires (halo.rs:187) from level l
 // and level l+1.  The reference runs fill_multires only from benches
@@ -545,6 +622,116 @@ void orc_sweep_multires(i64 sx, i64 sy, i64 sz, int L, float *fine_slab, const i
             off += cnt;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Halos of blocks at coarse levels (no reference counterpart: fill_multires,
+// halo.rs:187-358, only fills a fine centre block).  The centre block `bid`
+// sits at level l >= 1 (block edge E = 2^L[l]) of a multires grid whose level
+// k has dims[3k..3k+2] voxels, slab slabs[k] and slot table b2s[k] (block edge
+// halving per level, one block grid).  Tile cell (hx, hy, hz) of the HSX^3
+// tile (HSX = E + 2 halo) holds level-l voxel g = block origin + h - halo,
+// each axis resolved by the Neumann mirror of that level (boundary.rs:48-56);
+// the voxel's block b then decides where it is read, by levels[b]:
+//   * levels[b] == l: level l's own voxel;
+//   * levels[b] >  l: level l+1's voxel g >> 1 (the injection fill_multires
+//     uses for a coarser neighbour, halo.rs:263-268);
+//   * levels[b] <  l: the mean of level l-1's 2^3 voxels 2g + (dx, dy, dz),
+//     summed dz, dy, dx in order and divided by 8 (downsample_channel_avg,
+//     downsample.rs:44-53).
+// A block absent at the level read gives that level's empty value (slot 0).
+// ---------------------------------------------------------------------------
+static float level_read(const i64 *dims, const float *const *slabs, const i32 *const *b2s, int k, i32 x, i32 y,
+                        i32 z) {
+    const int L = 4 - k;
+    Level g(dims[3 * k], dims[3 * k + 1], dims[3 * k + 2], L, slabs[k], b2s[k]);
+    const i32 m = g.B - 1;
+    const i64 bid = (x >> L) + (i64)(y >> L) * g.nx + (i64)(z >> L) * g.nxy;
+    const i64 vid = (x & m) | ((i64)(y & m) << L) | ((i64)(z & m) << (2 * L));
+    return g.block_ptr(bid)[vid];
+}
+
+void orc_halo_fill_level(const i64 *dims, const float *const *slabs, const i32 *const *b2s, const uint8_t *levels,
+                         int n_levels, int l, i64 bid, int halo, float *tile) {
+    const int L = 4 - l, E = 1 << L, HSX = E + 2 * halo;
+    Level g(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2], L, slabs[l], b2s[l]);
+    const i32 bx0 = (i32)(bid % g.nx), by0 = (i32)((bid / g.nx) % g.ny), bz0 = (i32)(bid / g.nxy);
+    for (int hz = 0; hz < HSX; ++hz)
+        for (int hy = 0; hy < HSX; ++hy)
+            for (int hx = 0; hx < HSX; ++hx) {
+                i32 rx, ry, rz;
+                resolve_axis(bx0 * E + hx - halo, (i32)g.sx - 1, BC_NEUMANN, &rx);
+                resolve_axis(by0 * E + hy - halo, (i32)g.sy - 1, BC_NEUMANN, &ry);
+                resolve_axis(bz0 * E + hz - halo, (i32)g.sz - 1, BC_NEUMANN, &rz);
+                const i64 b = (rx >> L) + (i64)(ry >> L) * g.nx + (i64)(rz >> L) * g.nxy;
+                float v;
+                if (levels[b] == l || (levels[b] > l && l + 1 >= n_levels) || (levels[b] < l && l == 0)) {
+                    v = level_read(dims, slabs, b2s, l, rx, ry, rz);
+                } else if (levels[b] > l) {
+                    v = level_read(dims, slabs, b2s, l + 1, rx >> 1, ry >> 1, rz >> 1);
+                } else {
+                    float sum = 0.0f;
+                    for (int dz = 0; dz < 2; ++dz)
+                        for (int dy = 0; dy < 2; ++dy)
+                            for (int dx = 0; dx < 2; ++dx)
+                                sum += level_read(dims, slabs, b2s, l - 1, 2 * rx + dx, 2 * ry + dy, 2 * rz + dz);
+                    v = sum / 8.0f;
+                }
+                tile[((size_t)hz * HSX + hy) * HSX + hx] = v;
+            }
+}
+
+// The 8-colour sweep of a multires grid's blocks at all their levels: the
+// colour of a block is the parity of its block coordinates whatever its level
+// (solver.rs:247-267), and a colour pass updates every active block of that
+// colour -- at its own level, in place -- before the next colour starts.  A
+// level-0 block fills its halo with fill_multires (orc_halo_fill_multires,
+// levels 0 and 1); a block at level l >= 1 with orc_halo_fill_level.  Blocks
+// of one colour are never neighbours and a halo reaches only the 26
+// neighbours (halo <= E), so the order inside a colour does not matter.  The
+// kernels are the reference's (laplacian.rs, meancurv.rs, bilaplacian.rs) on
+// the level's block edge.  Blocks without a value slot at their level are
+// skipped (solver.rs:201-204).
+void orc_sweep_levels(const i64 *dims, float *const *slabs, const i32 *const *b2s, const uint8_t *levels,
+                      int n_levels, const i64 *active, i64 n, int stencil, float dt, int iterations, int clamp01,
+                      int nthreads) {
+    const int halo = stencil == 2 ? 2 : 1;
+    const int full = stencil == 0 ? 0 : 1;
+    Level g0(dims[0], dims[1], dims[2], 4, nullptr, nullptr);
+    std::vector<i64> by_color[8];
+    for (i64 i = 0; i < n; ++i) {
+        const i64 bid = active[i];
+        const i64 bx = bid % g0.nx, by = (bid / g0.nx) % g0.ny, bz = bid / g0.nxy;
+        by_color[(bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)].push_back(bid);
+    }
+    for (int it = 0; it < iterations; ++it)
+        for (int c = 0; c < 8; ++c) {
+            const std::vector<i64> &bucket = by_color[c];
+#pragma omp parallel num_threads(nthreads < 1 ? 1 : nthreads)
+            {
+                std::vector<float> tile(20 * 20 * 20);
+#pragma omp for schedule(dynamic, 16)
+                for (i64 k = 0; k < (i64)bucket.size(); ++k) {
+                    const i64 bid = bucket[k];
+                    const int l = levels[bid];
+                    const i32 s = b2s[l][bid];
+                    if (s < 2) continue;
+                    const int L = 4 - l, E = 1 << L;
+                    float *out = slabs[l] + (i64)s * E * E * E;
+                    if (l == 0)
+                        orc_halo_fill_multires(dims[0], dims[1], dims[2], 4, slabs[0], b2s[0], slabs[1], b2s[1],
+                                               levels, bid, halo, full, BC_NEUMANN, tile.data());
+                    else
+                        orc_halo_fill_level(dims, slabs, b2s, levels, n_levels, l, bid, halo, tile.data());
+                    if (stencil == 0)
+                        orc_kernel_laplacian(E, tile.data(), dt, clamp01, out);
+                    else if (stencil == 1)
+                        orc_kernel_meancurv(E, tile.data(), dt, clamp01, out);
+                    else
+                        orc_kernel_bilaplacian(E, tile.data(), dt, clamp01, out);
+                }
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------

@@ -277,9 +277,13 @@ void orc_g2p_blend(const float *u_old, const float *u, const float *u_tilde, i64
 // tests only place particles in blocks with bx = 0, where that region is
 // skipped (p2g.rs:367-369).
 // ---------------------------------------------------------------------------
-void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, const float *mass,
-                 const i64 *particle_blocks, const i64 *starts, i64 nblk, float r_p, float *slabs, i64 n_slots,
-                 const i32 *b2s, const i64 *active, i64 n_active, int ref_commit_width) {
+}  // extern "C"
+
+namespace {
+template <typename T>
+void p2g_mac_impl(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, const float *mass,
+                  const i64 *particle_blocks, const i64 *starts, i64 nblk, float r_p, T *slabs, i64 n_slots,
+                  const i32 *b2s, const i64 *active, i64 n_active, int ref_commit_width, bool normalize) {
     const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nz = (sz + 15) >> 4, nxy = nx * ny;
     const int m = (int)ceilf(r_p);
     const int MSX = ((16 + 2 * m) + 7) & ~7;  // staging extent, multiple of 8 (p2g.rs:174-180)
@@ -290,18 +294,18 @@ void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, con
         i64 bx = bid % nx, by = (bid / nx) % ny, bz = bid / nxy;
         by_color[(bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)].push_back({i, bid});
     }
-    std::vector<float> sm((size_t)MSX * MSX * MSX), sv((size_t)MSX * MSX * MSX);
+    std::vector<T> sm((size_t)MSX * MSX * MSX), sv((size_t)MSX * MSX * MSX);
     for (int a = 0; a < 3; ++a) {
-        float *vel_g = slabs + (i64)a * n_slots * 4096;
-        float *mass_g = slabs + (i64)(3 + a) * n_slots * 4096;
+        T *vel_g = slabs + (i64)a * n_slots * 4096;
+        T *mass_g = slabs + (i64)(3 + a) * n_slots * 4096;
         const float ofx = a == 0 ? 1.0f : 0.5f, ofy = a == 1 ? 1.0f : 0.5f, ofz = a == 2 ? 1.0f : 0.5f;
         for (int c = 0; c < 8; ++c)
             for (auto &ib : by_color[c]) {
                 const i64 i = ib.first, bid = ib.second;
                 const i32 bx = (i32)(bid % nx), by = (i32)((bid / nx) % ny), bz = (i32)(bid / nxy);
                 const i32 b0x = bx * 16, b0y = by * 16, b0z = bz * 16;
-                std::fill(sm.begin(), sm.end(), 0.0f);
-                std::fill(sv.begin(), sv.end(), 0.0f);
+                std::fill(sm.begin(), sm.end(), T(0));
+                std::fill(sv.begin(), sv.end(), T(0));
                 const i32 x0s = b0x - m, y0s = b0y - m, z0s = b0z - m;
                 const i32 window_hi = (16 + m) - 1;
                 for (i64 pi = starts[i]; pi < starts[i + 1]; ++pi) {  // p2g.rs:261-333
@@ -329,8 +333,8 @@ void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, con
                                 float d_a = a == 0 ? dx : (a == 1 ? dy : dz);
                                 float df2 = dist_sq + d_a + 0.25f;
                                 float w = wcubic_falloff(df2, inv_r2);
-                                sm[row + o] = sm[row + o] + w * scale_mass;
-                                sv[row + o] = sv[row + o] + w * scale_mom;
+                                sm[row + o] = sm[row + o] + (T)(w * scale_mass);  // the float product, summed in T
+                                sv[row + o] = sv[row + o] + (T)(w * scale_mom);
                             }
                         }
                     }
@@ -357,7 +361,7 @@ void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, con
                             if (gx0 + (xhi - xlo) > sx || gy0 + (yhi - ylo) > sy || gz0 + (zhi - zlo) > sz) continue;
                             i32 s = b2s[nbid];
                             if (s < 2) continue;
-                            float *md = mass_g + (i64)s * 4096, *vd = vel_g + (i64)s * 4096;
+                            T *md = mass_g + (i64)s * 4096, *vd = vel_g + (i64)s * 4096;
                             i32 lx0 = gx0 & 15, ly0 = gy0 & 15, lz0 = gz0 & 15;
                             // The reference's commit_sum adds W = 16 (interior) or 4
                             // (margin) lanes (p2g.rs:396-406) whatever the region
@@ -371,8 +375,8 @@ void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, con
                                     i64 drow = lx0 | ((ly0 + (syy - ylo)) << 4) | ((lz0 + (szz - zlo)) << 8);
                                     bool zm = true, zv = true;
                                     for (i32 k = 0; k < w; ++k) {
-                                        zm = zm && sm[srow + k] == 0.0f;
-                                        zv = zv && sv[srow + k] == 0.0f;
+                                        zm = zm && sm[srow + k] == T(0);
+                                        zv = zv && sv[srow + k] == T(0);
                                     }
                                     if (!zm)
                                         for (i32 k = 0; k < w; ++k) md[drow + k] = md[drow + k] + sm[srow + k];
@@ -382,20 +386,46 @@ void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, con
                         }
             }
     }
+    if (!normalize) return;
     // MacGrid::normalize_velocity (mac.rs:106-132).
     for (int a = 0; a < 3; ++a) {
-        float *vel_g = slabs + (i64)a * n_slots * 4096;
-        const float *mass_g = slabs + (i64)(3 + a) * n_slots * 4096;
+        T *vel_g = slabs + (i64)a * n_slots * 4096;
+        const T *mass_g = slabs + (i64)(3 + a) * n_slots * 4096;
         for (i64 k = 0; k < n_active; ++k) {
             i32 s = b2s[active[k]];
             if (s < 2) continue;
             for (int j = 0; j < 4096; ++j) {
-                float mm = mass_g[(i64)s * 4096 + j];
-                float &vv = vel_g[(i64)s * 4096 + j];
-                vv = mm > 0.0f ? vv / mm : 0.0f;
+                T mm = mass_g[(i64)s * 4096 + j];
+                T &vv = vel_g[(i64)s * 4096 + j];
+                vv = mm > T(0) ? vv / mm : T(0);
             }
         }
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_p2g_mac(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, const float *mass,
+                 const i64 *particle_blocks, const i64 *starts, i64 nblk, float r_p, float *slabs, i64 n_slots,
+                 const i32 *b2s, const i64 *active, i64 n_active, int ref_commit_width) {
+    p2g_mac_impl<float>(sx, sy, sz, pos, vel, mass, particle_blocks, starts, nblk, r_p, slabs, n_slots, b2s, active,
+                        n_active, ref_commit_width, true);
+}
+
+// The same splat with every window and grid cell accumulated in float64: each
+// contribution is the float32 product w * (m u_a) (or w * m) of the splat
+// above, and the sums carry no float32 rounding (relative error <= k 2^-53
+// for k terms).  Writes the unnormalised sums, slabs_f64 = [P_x, P_y, P_z,
+// M_x, M_y, M_z] (each n_slots * 4096), exact-width commit.  Checks a splat
+// whose float32 sums round differently from the reference's order (the
+// reference's own order is unspecified, particles/sort.rs:234-237).
+void orc_p2g_mac_f64(i64 sx, i64 sy, i64 sz, const float *pos, const float *vel, const float *mass,
+                     const i64 *particle_blocks, const i64 *starts, i64 nblk, float r_p, double *slabs_f64,
+                     i64 n_slots, const i32 *b2s, const i64 *active, i64 n_active) {
+    p2g_mac_impl<double>(sx, sy, sz, pos, vel, mass, particle_blocks, starts, nblk, r_p, slabs_f64, n_slots, b2s,
+                         active, n_active, 0, false);
 }
 
 // ---------------------------------------------------------------------------

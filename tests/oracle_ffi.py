@@ -40,6 +40,8 @@ def load():
     _sig(lib, "orc_sweep", None, i64, i64, i64, f32p, i32p, i64p, i64, i32, f32, i32, i32, i32, i32)
     _sig(lib, "orc_sweep_multires", None, i64, i64, i64, i32, f32p, i32p, f32p, i32p, u8p, i64p, i64, i32, f32,
          i32, i32, i32)
+    _sig(lib, "orc_halo_fill_level", None, i64p, C.c_void_p, C.c_void_p, u8p, i32, i32, i64, i32, f32p)
+    _sig(lib, "orc_sweep_levels", None, i64p, C.c_void_p, C.c_void_p, u8p, i32, i64p, i64, i32, f32, i32, i32, i32)
     _sig(lib, "orc_halo_topology", None, i64, i64, i64, i32, i32p, C.c_void_p, C.c_void_p, i64, i32, i32p)
     _sig(lib, "orc_downsample_avg", None, i32, f32p, i32p, f32p, i32p, i64p, i64)
     _sig(lib, "orc_sample_coarse", f32, i64, i64, i64, i32, f32p, i32p, i32, i32, i32)
@@ -56,6 +58,8 @@ def load():
     _sig(lib, "orc_p2g_mac", None, i64, i64, i64, f32p, f32p, f32p, i64p, i64p, i64, f32, f32p, i64, i32p,
          i64p, i64, i32)
     _sig(lib, "orc_p2g_mac_naive", None, i64, i64, i64, f32p, f32p, f32p, i64, f32, f32p)
+    _sig(lib, "orc_p2g_mac_f64", None, i64, i64, i64, f32p, f32p, f32p, i64p, i64p, i64, f32,
+         np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"), i64, i32p, i64p, i64)
     _sig(lib, "orc_bspline_weight", f32, i32, f32, i64)
     _sig(lib, "orc_p2g_bspline", None, i32, i64, i64, i64, f32p, f32p, C.c_void_p, i64, i32p, i64, i64p, i64, f32p)
     _sig(lib, "orc_g2p_bspline", None, i32, i64, i64, i64, f32p, i64, i32p, f32p, i64, f32p)
@@ -78,6 +82,10 @@ def load():
          i32)
     _sig(lib, "orc_sweep_density", None, i64, i64, i64, np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"), i64,
          i32p, i64p, i64, i32, f32, i32, i32)
+    _sig(lib, "orc_density8_uniform", f32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
+    _sig(lib, "orc_density8_code", C.c_uint8, f32, f32)
+    _sig(lib, "orc_sweep_density8", None, i64, i64, i64, u8p, i64, i32p, i64p, i64, i32, f32, i32, i32, C.c_uint32,
+         C.c_uint32)
     _sig(lib, "orc_net_param_count", i32)
     _sig(lib, "orc_net_forward", None, f32p, f32p, i64, i32, f32p)
     _sig(lib, "orc_net_loss_grad", C.c_double, f32p, f32p, f32p, i64, i32, f64p)

@@ -63,6 +63,33 @@ def test_jacobi_bitexact(pkg, orc, hip, s, seed, frac, st):
     assert_same_bits(dy.get(), oracle_jacobi(orc, g, active, xs, bs, 2.0 / 3.0, st), "jacobi")
 
 
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_jacobi_sweeps_graph_bitexact(pkg, orc, hip, st):
+    """msbg_poisson_jacobi_sweeps: N graph-replayed sweeps x -> tmp -> x equal
+    N oracle Jacobi applications; replayed twice back to back without a sync,
+    then on a second pair of buffers (the graph is re-captured while the
+    first may still run), and a different sweep count."""
+    g, active, x, b = band_problem(pkg, 64, 48, 80, 3, frac=0.7)
+    _, pb = setup(pkg, hip, g, active)
+    xs, bs = to_storage(x, st), to_storage(b, st)
+    db = hip.to_device(bs)
+    want = xs
+    for _ in range(20 + 20):
+        want = oracle_jacobi(orc, g, active, want, bs, 2.0 / 3.0, st)
+    dx, dt = hip.to_device(xs), hip.to_device(xs)
+    pb.jacobi_sweeps(st, dx, dt, db, 2.0 / 3.0, 20)
+    pb.jacobi_sweeps(st, dx, dt, db, 2.0 / 3.0, 20)
+    ex, et = hip.to_device(xs), hip.to_device(xs)  # new buffers: re-capture
+    pb.jacobi_sweeps(st, ex, et, db, 2.0 / 3.0, 20)
+    pb.jacobi_sweeps(st, ex, et, db, 2.0 / 3.0, 12)
+    hip.synchronize()
+    assert_same_bits(dx.get(), want, "40 graph sweeps")
+    want2 = xs
+    for _ in range(32):
+        want2 = oracle_jacobi(orc, g, active, want2, bs, 2.0 / 3.0, st)
+    assert_same_bits(ex.get(), want2, "20 + 12 graph sweeps on new buffers")
+
+
 @pytest.mark.parametrize("s,seed,frac", GRIDS)
 @pytest.mark.parametrize("st", [0, 1, 2])
 def test_rbgs_bitexact(pkg, orc, hip, s, seed, frac, st):

@@ -184,6 +184,61 @@ def test_multires_sweep_matches_oracle(pkg, orc, hip, stencil):
     assert_bitexact(df.to_host().slab, slab, "multires sweep")
 
 
+def level_pair(pkg, orc, hip, s, n_levels, seed):
+    from test_oracle_multires import LevelGrid
+    lg = LevelGrid(pkg, orc, s, n_levels, seed=seed, lo=0.0, hi=1.0)
+    dev = [pkg.DeviceGrid.from_host(g, lib=hip) for g in lg.grids]
+    return lg, dev, hip.to_device(lg.levels)
+
+
+@pytest.mark.parametrize("stencil,dt", [(0, 0.1), (1, 0.02), (2, 0.005)])
+@pytest.mark.parametrize("s,n_levels", [((160, 128, 128), 3), ((160, 128, 128), 4), ((40, 56, 72), 4)])
+def test_level_sweep_matches_oracle(pkg, orc, hip, stencil, dt, s, n_levels):
+    """All-level sweeps (every block at its own level; coarse halos read finer
+    neighbours as 2^3 means and coarser ones by injection) bit for bit against
+    the oracle, including partial blocks at every level ((40, 56, 72))."""
+    lg, dev, dl = level_pair(pkg, orc, hip, s, n_levels, seed=stencil + 3 * n_levels + s[0])
+    active = np.arange(lg.levels.size)
+    pkg.sweep_levels(dev, dl, active, stencil, pkg.PdeParams(dt, 2, stencil == 0))
+    lg.sweep(orc, active, stencil, dt, 2, stencil == 0)
+    for l in range(n_levels):
+        assert_bitexact(dev[l].to_host().slab, lg.slabs[l], "level %d" % l)
+
+
+def test_level_sweep_graph_alternating(pkg, orc, hip):
+    """A plan over a subset of the blocks of all levels, MC19 and BiLap25
+    graphs replayed alternately without synchronisation, against the oracle."""
+    lg, dev, dl = level_pair(pkg, orc, hip, (160, 128, 128), 4, seed=5)
+    rng = np.random.default_rng(2)
+    active = np.nonzero(rng.random(lg.levels.size) < 0.8)[0]
+    plan = pkg.SweepPlan(dev[0], active, levels=dl, level_grids=dev)
+    keys = [(1, 0.02), (2, 0.005)] * 3
+    for st, dt in keys:
+        plan.run_graph_async(st, dt, 1)
+    hip.synchronize()
+    for st, dt in keys:
+        lg.sweep(orc, active, st, dt, 1)
+    for l in range(4):
+        assert_bitexact(dev[l].to_host().slab, lg.slabs[l], "level %d" % l)
+
+
+def test_level_sweep_five_levels(pkg, orc, hip):
+    """Five levels: edge-1 blocks at level 4 take the radius-1 stencils; the
+    radius-2 bi-Laplacian is refused with an error, and nothing is written."""
+    lg, dev, dl = level_pair(pkg, orc, hip, (160, 128, 128), 5, seed=9)
+    assert (lg.levels == 4).sum() > 10
+    active = np.arange(lg.levels.size)
+    for st, dt in ((0, 0.1), (1, 0.02)):
+        pkg.sweep_levels(dev, dl, active, st, pkg.PdeParams(dt, 1, False))
+        lg.sweep(orc, active, st, dt, 1)
+    for l in range(5):
+        assert_bitexact(dev[l].to_host().slab, lg.slabs[l], "level %d" % l)
+    with pytest.raises(pkg.MsbgError):
+        pkg.sweep_levels(dev, dl, active, 2, pkg.PdeParams(0.005, 1, False))
+    for l in range(5):
+        assert_bitexact(dev[l].to_host().slab, lg.slabs[l], "level %d after the refused call" % l)
+
+
 def test_downsample_matches_oracle(pkg, orc, hip):
     fine, coarse, levels = multires_pair(pkg, 3)
     bids = np.nonzero(levels == 0)[0]
@@ -256,6 +311,61 @@ def test_graph_replay_matches_eager(pkg, orc, hip, stencil):
         assert_bitexact(gb.to_host().slab, ga.to_host().slab, "replay %d" % rep)
     orc.sweep(g, active, stencil, 0.02, 6, True)
     assert_bitexact(gb.to_host().slab, g.slab, "graph vs oracle")
+
+
+@pytest.mark.parametrize("stencil,dt", [(0, 0.1), (1, 0.02), (2, 0.005)])
+@pytest.mark.parametrize("s", [(40, 33, 48), (64, 48, 80)])
+def test_density8_sweep_bitexact(pkg, orc, hip, stencil, dt, s):
+    """Sweeper<Density8>: u8 storage, linear dequantised halo, sqrt-compressed
+    stochastic-rounding store with the stated (seed, sweep, bid, voxel)
+    stream; a one-shot call (sweeps 0, 1) and a plan run eagerly once then
+    replayed as a graph twice (sweeps 0, 1, 2) against the oracle."""
+    import ctypes as C
+    g = random_grid(pkg, *s, seed=21 + stencil, lo=0.0, hi=1.0, empty=0.0, full=1.0)
+    active = np.nonzero(g.b2s >= 2)[0].astype(np.int64)
+    rng = np.random.default_rng(stencil)
+    q = rng.integers(0, 256, size=(g.n_slots, 4096)).astype(np.uint8)
+    q[0], q[1] = 0, 255
+    layout = pkg.DeviceGrid.from_host(g, lib=hip)
+    want = q.copy()
+    orc.orc_sweep_density8(g.sx, g.sy, g.sz, want, g.n_slots, g.b2s, active, active.size, stencil, dt, 2, 1, 77, 0)
+    d = hip.to_device(q)
+    hip.call("msbg_sweep_density8", layout.h, d.ptr, active.ctypes.data, active.size, stencil, dt, 2, 1, 77)
+    assert np.array_equal(d.get(), want)
+    want3 = q.copy()
+    orc.orc_sweep_density8(g.sx, g.sy, g.sz, want3, g.n_slots, g.b2s, active, active.size, stencil, dt, 3, 1, 78, 0)
+    d2 = hip.to_device(q)
+    plan = C.c_void_p()
+    hip.call("msbg_sweep_plan_new_density8", layout.h, d2.ptr, active.ctypes.data, active.size, 78, C.byref(plan))
+    hip.call("msbg_sweep_plan_run_async", plan, stencil, dt, 1, 1)
+    hip.call("msbg_sweep_plan_run_graph_async", plan, stencil, dt, 1, 1)
+    hip.call("msbg_sweep_plan_run_graph_async", plan, stencil, dt, 1, 1)
+    hip.synchronize()
+    hip.call("msbg_sweep_plan_free", plan)
+    assert np.array_equal(d2.get(), want3)
+
+
+def test_graph_cache_alternating_stencils_without_sync(pkg, orc, hip):
+    """A multires plan replays MC19 and BiLap25 graphs back to back with no
+    synchronisation in between (the benchmark's step), then a fifth key forces
+    an eviction while earlier graphs may still run: every result equals the
+    oracle's sweeps in the same order, bit for bit."""
+    fine, coarse, levels = multires_pair(pkg, 23)
+    fine._slab[2:] = np.abs(fine._slab[2:])
+    df, dc = pkg.DeviceGrid.from_host(fine, lib=hip), pkg.DeviceGrid.from_host(coarse, lib=hip)
+    dl = hip.to_device(levels)
+    active = np.nonzero(levels == 0)[0]
+    plan = pkg.SweepPlan(df, active, coarse=dc, levels=dl)
+    keys = [(1, 0.02, 1, False), (2, 0.005, 1, False)] * 4 + [(0, 0.1, 1, True), (1, 0.03, 2, False),
+                                                              (2, 0.004, 1, True), (1, 0.02, 1, False)]
+    for st, dt, it, cl in keys:
+        plan.run_graph_async(st, dt, it, cl)
+    hip.synchronize()
+    slab = np.ascontiguousarray(fine.slab)
+    for st, dt, it, cl in keys:
+        orc.orc_sweep_multires(fine.sx, fine.sy, fine.sz, 4, slab, fine.b2s, coarse.slab, coarse.b2s, levels,
+                               active, active.size, st, dt, it, int(cl), 1)
+    assert_bitexact(df.to_host().slab, slab, "alternating graph replays")
 
 
 @pytest.mark.parametrize("stencil,dt", [(0, 0.1), (1, 0.02), (2, 0.005)])

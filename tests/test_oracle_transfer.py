@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from transfer_helpers import (dense_from_slabs, oracle_g2p_bspline, oracle_p2g_bspline, oracle_p2g_mac,
-                              scatter_bspline_f64)
+                              oracle_p2g_mac_f64, scatter_bspline_f64)
 
 S32 = (32, 32, 32)
 
@@ -45,6 +45,81 @@ def test_staged_p2g_many_particles_matches_naive(orc):
     b = naive(orc, s, pos, vel, mass, 2.0)
     np.testing.assert_allclose(a[3:], b[3:], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(a[:3], b[:3], rtol=1e-4, atol=1e-4)
+
+
+def dense_f64_brute_force(s, pos, vel, mass, r_p):
+    """Eq. 6 splat as a dense per-particle scatter: every face cell of the
+    domain within the particle's footprint box gets the float32 products
+    w * m and w * (m u_a), summed in float64 (no windows, regions or commit).
+    The weight is the staged splat's float32 expression (fluid/p2g.rs:300-316):
+    dx = x - (p - 1/2), |d|^2 = dx^2 + (dz^2 + dy^2), df2 = |d|^2 + d_a + 1/4."""
+    sx, sy, sz = s
+    f = np.float32
+    out = np.zeros((6, sz, sy, sx))
+    r = f(r_p)
+    inv_r2 = f(1.0) / (r * r)
+    for p, u, m in zip(np.asarray(pos, f), np.asarray(vel, f), np.asarray(mass, f)):
+        p0 = p - f(0.5)
+        for a in range(3):
+            of = np.array([f(1.0) if a == k else f(0.5) for k in range(3)], f)
+            lo = np.maximum(np.ceil(p - r - of).astype(int), 0)
+            hi = np.minimum(np.floor(p + r - of).astype(int), np.array([sx, sy, sz]) - 1)
+            if (hi < lo).any():
+                continue
+            ix, iy, iz = (np.arange(lo[k], hi[k] + 1) for k in range(3))
+            Z, Y, X = np.meshgrid(iz, iy, ix, indexing="ij")
+            dx, dy, dz = X.astype(f) - p0[0], Y.astype(f) - p0[1], Z.astype(f) - p0[2]
+            d2 = dx * dx + (dz * dz + dy * dy)
+            df2 = d2 + (dx, dy, dz)[a] + f(0.25)
+            w = np.maximum(f(1.0) - df2 * inv_r2, f(0.0))
+            w = w * w * w
+            sl = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+            out[3 + a][sl] += (w * m).astype(np.float64)
+            out[a][sl] += (w * (m * u[a])).astype(np.float64)
+    return out
+
+
+@pytest.mark.parametrize("r_p", [1.5, 2.0, 3.0])
+def test_f64_splat_matches_dense_brute_force(orc, r_p):
+    """The float64-accumulated staged splat (windows, colour order, 27-region
+    commit) equals a dense per-particle scatter of the same products to
+    float64 rounding, on a whole-block domain where the commit drops only
+    cells outside the domain; a dropped region or a misplaced cell would
+    differ by a whole contribution."""
+    rng = np.random.default_rng(int(r_p * 10))
+    s = (48, 32, 48)
+    n = 300
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(n, 3)).astype(np.float32)
+    pos[:3] = [[15.9, 8.0, 8.0], [16.1, 15.95, 31.99], [0.05, 31.9, 47.9]]  # block faces, domain corners
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    _, b2s, sums = oracle_p2g_mac_f64(orc, s, pos, vel, mass, r_p)
+    got = np.stack([dense_from_slabs(b2s, sums[c], s) for c in range(6)])
+    want = dense_f64_brute_force(s, pos, vel, mass, r_p)
+    scale = dense_f64_brute_force(s, pos, np.abs(vel), mass, r_p)
+    assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300)
+    assert (want[3:] > 0).sum() > 10000
+
+
+def test_f64_splat_within_float_reordering_of_staged(orc):
+    """The float32 staged splat (the reference's order) is within the
+    reordering bound of its float64 sums: for k positive terms a float32 sum
+    is within (k - 1) 2^-24 of the exact one."""
+    rng = np.random.default_rng(5)
+    s = (48, 48, 48)
+    n = 3000
+    pos = rng.uniform(0.0, 48 - 1e-3, size=(n, 3)).astype(np.float32)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    _, b2s, f32s = oracle_p2g_mac(orc, s, pos, vel, mass, 2.0)
+    _, b2s64, sums = oracle_p2g_mac_f64(orc, s, pos, vel, mass, 2.0)
+    assert np.array_equal(b2s, b2s64)
+    ones = oracle_p2g_mac_f64(orc, s, pos, vel, np.ones(n, np.float32), 2.0)[2]
+    for a in range(3):
+        M, M64 = f32s[3 + a], sums[3 + a]
+        k = np.maximum((ones[3 + a] > 0) * 64.0, 1.0)  # <= 64 particles reach a cell here
+        assert np.all(np.abs(M - M64) <= k * 2.0 ** -24 * M64 + 1e-30)
+        assert np.abs(M - M64).max() > 0  # the float32 order does round
 
 
 def test_reference_margin_commit_spill(orc):

@@ -24,6 +24,29 @@ def oracle_p2g_mac(orc, s, pos, vel, mass, r_p, ref_commit_width=0):
     return active, b2s, slabs
 
 
+def oracle_p2g_mac_f64(orc, s, pos, vel, mass, r_p):
+    """The same splat summed in float64: returns (active, b2s, sums[6, n_slots,
+    4096]) = [P_x, P_y, P_z, M_x, M_y, M_z], unnormalised."""
+    sx, sy, sz = s
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    mass = np.ascontiguousarray(mass, np.float32)
+    n = pos.shape[0]
+    nb = ((sx + 15) >> 4) * ((sy + 15) >> 4) * ((sz + 15) >> 4)
+    bids = np.zeros(n, np.int64)
+    orc.orc_particle_bids(sx, sy, sz, pos, n, bids)
+    pb, starts, perm = orc.bucket_indices(bids, nb)
+    active = orc.active_blocks(sx, sy, sz, pos, r_p + 1.0)
+    b2s = np.zeros(nb, np.int32)
+    b2s[active] = np.arange(2, 2 + active.size, dtype=np.int32)
+    n_slots = 2 + active.size
+    sums = np.zeros((6, n_slots, 4096), np.float64)
+    orc.orc_p2g_mac_f64(sx, sy, sz, np.ascontiguousarray(pos[perm]), np.ascontiguousarray(vel[perm]),
+                        np.ascontiguousarray(mass[perm]), pb, starts, pb.size, r_p, sums, n_slots, b2s, active,
+                        active.size)
+    return active, b2s, sums
+
+
 def dense_from_slabs(b2s, slab, s):
     """(sz, sy, sx) dense view of one channel."""
     sx, sy, sz = s
