@@ -78,12 +78,36 @@ def oracle_cg64_solution(orc, g, bids, b):
     return x.reshape(-1, 4096), it
 
 
+def host_cpu():
+    """Model name, logical CPUs of the host, and the CPUs this process may run
+    on (its affinity mask)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
+
+
+# One GPU's share of a GPU box's host CPUs: the box runs up to 16 worker
+# threads per GPU, whatever nproc reports.
+CPU_SHARE = 16
+
+
 def cpu_baseline(sc, dt_mc, dt_bl, reps, solves):
     """CPU leg.  Baseline: the oracle's C++ restatement of the reference
-    sweeper (thread-local halo tiles, OpenMP over the blocks of a colour; FTZ /
-    DAZ off, no denormals arise in this field).  Also the FP64 CG oracle
-    solutions the GPU solves are compared with (`solves`: label -> (grid,
-    blocks, rhs slab, {variant: x})."""
+    sweeper (thread-local halo tiles, OpenMP over the blocks of a colour, FTZ /
+    DAZ on in every worker as the reference's benchmark pool sets it), on the
+    host CPUs this process may use, at most one GPU's share of them.  Also the
+    FP64 CG oracle solutions the GPU solves are compared with (`solves`:
+    label -> (grid, blocks, rhs slab, {variant: x})."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
 
@@ -95,8 +119,10 @@ def cpu_baseline(sc, dt_mc, dt_bl, reps, solves):
         log("fp64 oracle %s: %.1fs" % (label, time.time() - t))
     fine, coarse = sc.grids[0], sc.grids[1]
     active = sc.level_blocks(0)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, n_host, n_allowed = host_cpu()
+    threads = max(1, min(CPU_SHARE, n_allowed))
     slab = np.ascontiguousarray(fine.slab).copy()
+    orc.orc_set_multires_ftz(1)
     best = None
     for r in range(reps + 1):
         t = time.perf_counter()
@@ -106,10 +132,15 @@ def cpu_baseline(sc, dt_mc, dt_bl, reps, solves):
         el = time.perf_counter() - t
         if r > 0:
             best = el if best is None else min(best, el)
+    orc.orc_set_multires_ftz(0)
     cells = active.size * 4096 * 2
     return {"value": cells / best, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_logical_cpus": n_host, "process_cpus": n_allowed,
             "sample": "full workload: one MC19 + one BiLap25 8-colour multires sweep over %d level-0 blocks, "
-                      "best of %d after 1 warm-up, FTZ/DAZ off" % (active.size, reps)}, errors
+                      "best of %d after 1 warm-up, FTZ/DAZ on, OpenMP on %d threads (one GPU's share of the "
+                      "host: at most %d; %d logical CPUs on the host, %d in this process's affinity mask), "
+                      "C++ restatement of the msbg-rs CPU sweeper built -O3 -march=x86-64-v3"
+                      % (active.size, reps, threads, CPU_SHARE, n_host, n_allowed)}, errors
 
 
 def particle_leg(pkg, lib, sc, n, steps, warmup):
@@ -353,14 +384,13 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
     cap = int(active_blocks(S, S, S, dp, 3.0, lib=lib).size * 1.1) + 64  # above the P2G footprint (r_p = 2)
     fs = FlipStep(S, S, S, capacity_blocks=cap, lib=lib)
     out = {"domain": S, "particles": n, "capacity_blocks": cap}
-    for name, storage, method in (("f32_plain", 0, "sor"), ("f16_refine", 1, "refine"), ("f32_mg", 0, "mg"),
-                                  ("f16_mg", 1, "mg")):
+    for name, storage, method, comp in FLIP_VARIANTS:
         args = solve_args(method)
         if method != "mg":
             args.update(sweeps_per_check=10, max_sweeps=max_sweeps)
         best = None
         for _ in range(reps):
-            _, _, st = fs.run(dp, dv, dm, r_p=2.0, alpha=0.95, storage=storage, tol=1e-6, **args)
+            _, _, st = fs.run(dp, dv, dm, r_p=2.0, alpha=0.95, storage=storage, tol=1e-6, compensated=comp, **args)
             if best is None or st["ms"]["total"] < best["ms"]["total"]:
                 best = st
         layout, band, p, b, active, band_blocks = fs.last
@@ -376,12 +406,56 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
                      "particles_per_s_transfers": n / ((best["ms"]["sort"] + best["ms"]["p2g"] + best["ms"]["g2p"]) * 1e-3)}
         log("flip %s: %.1f ms total, %d sweeps, rel %.2e" % (name, best["ms"]["total"], sv["sweeps"],
                                                              sv["rel_residual"]))
-    for k in ("f16_refine", "f32_mg", "f16_mg"):
+    for k in [v[0] for v in FLIP_VARIANTS[1:]]:
         out[k + "_vs_f32_total"] = out[k]["ms"]["total"] / out["f32_plain"]["ms"]["total"]
     out["note"] = ("single resolution level at S^3; the pressure band is the MAC blocks (those the particles' P2G "
                    "footprint touches) whose -x/-y/-z neighbours are MAC blocks, p = 0 outside it; div_before / "
                    "div_after are L2 norms over the band's cells; an FP32 pressure stalls near 2.3e-7 ||p||/||b||, "
-                   "so the solve stops at the sweep cap when that floor is above 1e-6")
+                   "so the solve stops at its floor when that is above 1e-6; *_pair carries the pressure as the "
+                   "compensated sum of two FP32 fields through the solve and the projection")
+    return out
+
+
+# (name, storage, method, compensated pressure)
+FLIP_VARIANTS = (("f32_plain", 0, "sor", False), ("f16_refine", 1, "refine", False), ("f32_mg", 0, "mg", False),
+                 ("f16_mg", 1, "mg", False), ("f16_mg_pair", 1, "mg", True))
+
+
+def flip_fp64_leg(pkg, lib, orc, S, n):
+    """Final-field error of the FLIP step's pressure against the FP64 CG
+    oracle (CG to ||r|| <= 1e-11 ||b|| on the host), on the dam-break scene at
+    S^3 with n particles: the 1024^3 / 64M band (124M unknowns) is beyond a CPU
+    CG in minutes, so the same generator and step run at S^3, n = 64M (S /
+    1024)^3.  Each variant's p (+ p_lo) is compared over the band's cells."""
+    from msbg_rs_amd import scenes
+    from msbg_rs_amd.flip import FlipStep
+    pos, vel, mass = scenes.dam_break(S, n)
+    dp, dv, dm = lib.to_device(pos), lib.to_device(vel), lib.to_device(mass)
+    fs = FlipStep(S, S, S, capacity_blocks=(S // 16) ** 3, lib=lib)
+    out, ref = {"domain": S, "particles": n}, None
+    for name, storage, method, comp in FLIP_VARIANTS:
+        args = solve_args(method)
+        if method != "mg":
+            args.update(sweeps_per_check=10, max_sweeps=20000)
+        _, _, st = fs.run(dp, dv, dm, r_p=2.0, alpha=0.95, storage=storage, tol=1e-6, compensated=comp, **args)
+        layout, band, p, b, active, band_blocks = fs.last
+        if ref is None:
+            L = layout.to_host()
+            bh = b.get().reshape(-1, 4096)
+            t = time.time()
+            xd, it = oracle_cg64_solution(orc, L, band_blocks, bh)
+            rows = L.b2s[band_blocks]
+            ref = xd[rows]
+            out["cells"] = int(band.n_cells)
+            out["cg64_iterations"] = it
+            log("fp64 oracle flip %d^3: %.1fs, %d iterations" % (S, time.time() - t, it))
+        x = p.get().reshape(-1, 4096)[rows].astype(np.float64)
+        if comp:
+            x = x + fs.p_lo.get().reshape(-1, 4096)[rows].astype(np.float64)
+        d = x - ref
+        out[name] = {"rel_residual": st["solve"]["rel_residual"], "converged": st["solve"]["converged"],
+                     "max_rel": float(np.abs(d).max() / np.abs(ref).max()),
+                     "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(ref))}
     return out
 
 
@@ -435,7 +509,12 @@ def surface_leg(pkg, lib, res, n_base, n_inst, reps=3):
 SOLVE_VARIANTS = (("f32_plain", 0, "sor", False), ("f32_learned", 0, "sor", True), ("f16_refine", 1, "refine", False),
                   ("bf16_refine", 2, "refine", False), ("f16_refine_learned", 1, "refine", True),
                   ("bf16_refine_learned", 2, "refine", True), ("f32_mg", 0, "mg", False), ("f16_mg", 1, "mg", False),
-                  ("bf16_mg", 2, "mg", False))
+                  ("bf16_mg", 2, "mg", False), ("f16_mg_learned", 1, "mg", True))
+
+
+# --direct-solves: every solve launches its check periods directly instead of
+# replaying captured graphs (profiled runs).
+SOLVE_DIRECT = False
 
 
 def solve_args(method):
@@ -443,8 +522,8 @@ def solve_args(method):
     residual check every 4 sweeps; multigrid V(2, 2)-cycles (Gauss-Seidel
     smoothing) with a check after every cycle."""
     if method == "mg":
-        return dict(multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 100)
-    return dict(refine=method == "refine", omega=1.9, sweeps_per_check=4, max_sweeps=20000)
+        return dict(multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4 * 100, direct=SOLVE_DIRECT)
+    return dict(refine=method == "refine", omega=1.9, sweeps_per_check=4, max_sweeps=20000, direct=SOLVE_DIRECT)
 
 
 def solve_leg(pkg, lib, band, phi, b, reps, keep_x=False):
@@ -528,11 +607,17 @@ def main():
                     help="domain edge of the FLIP-step leg (dam-break box + droplets; 0 = skip)")
     ap.add_argument("--flip-particles", type=int, default=64 << 20)
     ap.add_argument("--flip-sweeps", type=int, default=6000, help="sweep cap of the FLIP pressure solve")
+    ap.add_argument("--flip-fp64", type=int, default=256, metavar="S",
+                    help="domain edge of the FLIP step compared with the FP64 CG oracle (0 = skip)")
     ap.add_argument("--skip-solves", action="store_true",
                     help="stop before the 128^3 shell, 512^3 band-solve and CPU legs (kernel profiles)")
-    ap.add_argument("--fp64-band", action="store_true",
-                    help="also compare the 512^3 band solves with the FP64 CG oracle (minutes of CPU)")
+    ap.add_argument("--no-fp64-band", action="store_true",
+                    help="skip the FP64 CG oracle comparison of the 512^3 band solves (~10-20 s of CPU)")
+    ap.add_argument("--direct-solves", action="store_true",
+                    help="launch the solves' kernels directly instead of replaying captured graphs (profiled runs)")
     args = ap.parse_args()
+    global SOLVE_DIRECT
+    SOLVE_DIRECT = args.direct_solves
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the sparse-grid path is not sharded across GPUs "
               "(run with --gpus 1)", file=sys.stderr)
@@ -668,16 +753,21 @@ def main():
     rhs512 = scenes.poisson_rhs(sc.grids[0], active)
     band512 = pkg.PoissonBand(fine, active, lib=lib)
     solve512, xs512 = solve_leg(pkg, lib, band512, fine, lib.to_device(rhs512), args.solve_reps,
-                                keep_x=args.fp64_band and not args.no_cpu)
+                                keep_x=not args.no_fp64_band and not args.no_cpu)
     out["solve_512"] = {"blocks": int(active.size), "cells": band512.n_cells, "variants": solve512,
                         "note": "7-point Poisson on the level-0 band, zero Dirichlet outside it, LCG right-hand "
                                 "side; red-black SOR omega 1.9, FP64 residual check every 4 sweeps"}
     if not args.no_cpu:
         log("cpu baseline ...")
         solves = {"shell_128": (g1, bids1, scenes.poisson_rhs(g1, bids1), xs1)}
-        if args.fp64_band:
+        if not args.no_fp64_band:
             solves["band_512"] = (sc.grids[0], active, rhs512, xs512)
         out["cpu_baseline"], out["fp64_error"] = cpu_baseline(sc, dt_mc, dt_bl, args.cpu_reps, solves)
+        if args.flip_fp64 > 0:
+            log("FLIP FP64 oracle leg ...")
+            import oracle_ffi
+            n = int((64 << 20) * (args.flip_fp64 / 1024.0) ** 3)
+            out["fp64_error"]["flip"] = flip_fp64_leg(pkg, lib, oracle_ffi.load(), args.flip_fp64, n)
     print(json.dumps(out), flush=True)
 
 

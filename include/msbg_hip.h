@@ -70,7 +70,7 @@ int msbg_event_elapsed_ms(void *start, void *stop, float *ms);
  * (blockpool.rs:118-131). */
 typedef struct msbg_grid msbg_grid;
 
-/* SparseGrid::new -- sparse_grid.rs:107-155 (log2_bsx 1..4). */
+/* SparseGrid::new -- sparse_grid.rs:107-155 (log2_bsx 0..4: edge 16 >> l of multires level l, grid.rs:57-63). */
 int msbg_grid_new(int64_t sx, int64_t sy, int64_t sz, int log2_bsx, float empty_value, float full_value,
                   int64_t capacity_blocks, msbg_grid **out);
 int msbg_grid_free(msbg_grid *g);
@@ -334,6 +334,11 @@ int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *ac
  * divergence equals div - A p, the solve's residual. */
 int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
                                const int64_t *active, int64_t n, float scale);
+/* The same projection from a compensated pressure p + p_lo (the x, x_lo pair
+ * of a msbg_poisson_solve with x_lo): the gradient is (p(c + e_a) - p(c)) +
+ * (p_lo(c + e_a) - p_lo(c)).  No reference counterpart. */
+int msbg_mac_subtract_gradient_pair(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
+                                    const float *d_p_lo, const int64_t *active, int64_t n, float scale);
 /* particles::splat -- particles/splat.rs:92-319 with D = Density (u16,
  * channel.rs:303-375): the active blocks of the particles (footprint r_scan,
  * active.rs:15-47) are ensured in the layout grid g and filled with 65535 in
@@ -391,6 +396,7 @@ int msbg_density_finalize(const msbg_grid *layout, uint16_t *d_slab16, int64_t s
  *   x_c' = fmaf(omega / 6, t, x_c) rounded to the storage type (RNE).
  * Storage types: FP32, IEEE binary16, bfloat16; arithmetic is FP32. */
 enum { MSBG_STORAGE_F32 = 0, MSBG_STORAGE_F16 = 1, MSBG_STORAGE_BF16 = 2 };
+enum { MSBG_SOLVE_DIRECT = 1 };
 typedef struct msbg_poisson msbg_poisson;
 typedef struct msbg_learned msbg_learned;
 typedef struct {
@@ -411,10 +417,18 @@ typedef struct {
     float omega;          /* SOR factor, 0 < omega < 2 */
     int sweeps_per_check; /* red-black sweeps between FP64 residual checks (inner sweeps per refinement step) */
     int max_sweeps;       /* cap on the band's red-black sweeps (refine = 2: 2 sweeps_per_check per cycle) */
-    int reserved;         /* 0 */
+    int flags;            /* 0, or MSBG_SOLVE_DIRECT: launch every check period's kernels directly on the
+                             library stream instead of replaying a captured HIP graph (for callers that
+                             capture the solve themselves, and for profilers that trace graph launches
+                             poorly); same kernels, same results */
     double tol;           /* stop when ||b - A x||_2 <= tol * ||b||_2 (FP64 norms) */
     msbg_learned *learned;   /* NULL, or a network proposing a correction of x after every check period */
     const msbg_grid *phi;    /* phase field (network input; layout dimensions, 16^3 blocks) */
+    float *x_lo;          /* NULL, or (refine = 1 / 2, no learned correction) an FP32 field holding the low
+                             part of a compensated iterate x + x_lo (zero, or a previous solve's low part, on
+                             entry): the refinement updates the pair with error-free FP32 sums and computes
+                             r = b - A (x + x_lo) compensated, so the residual can fall below the floor one
+                             FP32 field sets (~2.3e-7 ||x|| / ||b||); on return x = round(x + x_lo) */
 } msbg_poisson_params;
 typedef struct {
     int sweeps;           /* red-black sweeps done */
@@ -446,6 +460,11 @@ int msbg_poisson_rbgs(msbg_poisson *p, int storage, void *x, const void *b, int 
  * norms of r and b (synchronous). */
 int msbg_poisson_residual(msbg_poisson *p, int storage, const void *x, const void *b, float *r, double *r2,
                           double *b2);
+/* r = b - A (x + x_lo) of a compensated FP32 iterate (see x_lo below), rounded
+ * once to FP32 (r: slab or NULL), and the FP64 squared norms of r and b
+ * (synchronous).  No reference counterpart. */
+int msbg_poisson_residual_pair(msbg_poisson *p, const float *x, const float *x_lo, const float *b, float *r,
+                               double *r2, double *b2);
 /* Element-wise storage conversion of n_elems values (RNE on narrowing). */
 int msbg_poisson_convert(int src_storage, const void *src, int dst_storage, void *dst, int64_t n_elems);
 /* Solve A x = b (x, b: FP32 slabs; x holds the initial guess) with red-black

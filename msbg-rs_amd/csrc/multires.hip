@@ -132,7 +132,8 @@ int msbg_downsample_avg(msbg_grid *coarse, const msbg_grid *fine, const int64_t 
         switch (fine->L) {
             case 4: hipLaunchKernelGGL(downsample_kernel<4>, grid, block, 0, stream(), f, c, d_bids); break;
             case 3: hipLaunchKernelGGL(downsample_kernel<3>, grid, block, 0, stream(), f, c, d_bids); break;
-            default: hipLaunchKernelGGL(downsample_kernel<2>, grid, block, 0, stream(), f, c, d_bids); break;
+            case 2: hipLaunchKernelGGL(downsample_kernel<2>, grid, block, 0, stream(), f, c, d_bids); break;
+            default: hipLaunchKernelGGL(downsample_kernel<1>, grid, block, 0, stream(), f, c, d_bids); break;
         }
         e = hipGetLastError();
     }

@@ -49,6 +49,11 @@ __global__ void run_write_kernel(const uint32_t *__restrict__ keys, int64_t n, c
 
 __global__ void set_i32_kernel(int32_t *p, int32_t v) { *p = v; }
 
+__global__ void iota_fill_kernel(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // footprint_axis (particles/mod.rs:207-212): trunc(p/bsx -+ r_scan/bsx) clipped.
 __device__ __forceinline__ void footprint(float p, float rb, int32_t nb, int32_t &lo, int32_t &hi) {
     const float b = p / 16.0f;
@@ -107,6 +112,28 @@ int run_lengths(const uint32_t *keys, int64_t n, int32_t *blocks, int32_t *start
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     *n_occ = total;
+    return MSBG_OK;
+}
+
+// Marks, scans and compacts the footprint blocks of n particles into d_out
+// (ascending); flags / rank hold nb words each, sums scan_sums_words(nb).
+// The flags are cleared by a fill kernel, not hipMemsetAsync, so the whole
+// sequence is plain kernel launches plus the count's read-back.
+int active_blocks_into(const float *d_pos, int64_t n, float r_scan, int32_t nx, int32_t ny, int32_t nz,
+                       uint32_t *flags, uint32_t *rank, uint32_t *sums, int32_t *d_out, int64_t *n_out) {
+    const int64_t nb = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(iota_fill_kernel, dim3(grid_for(nb)), dim3(256), 0, stream(), flags, nb, 0u);
+    if (n > 0)
+        hipLaunchKernelGGL(mark_active_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, r_scan / 16.0f, nx,
+                           ny, nz, flags);
+    MSBG_HIP(hipGetLastError());
+    uint32_t total = 0;
+    int rc = scan_exclusive_u32_with(flags, rank, nb, sums, &total);
+    if (rc) return rc;
+    hipLaunchKernelGGL(compact_flags_kernel, dim3(grid_for(nb)), dim3(256), 0, stream(), flags, rank, nb, d_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    *n_out = total;
     return MSBG_OK;
 }
 
@@ -197,20 +224,9 @@ int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, i
     int rc;
     uint32_t *flags = (uint32_t *)workspace(5, sizeof(uint32_t) * 2 * nb, &rc);
     if (rc) return rc;
-    uint32_t *rank = flags + nb;
-    MSBG_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * nb, stream()));
-    if (n > 0)
-        hipLaunchKernelGGL(mark_active_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, r_scan / 16.0f, nx,
-                           ny, nz, flags);
-    MSBG_HIP(hipGetLastError());
-    uint32_t total = 0;
-    rc = scan_exclusive_u32(flags, rank, nb, &total);
+    uint32_t *sums = (uint32_t *)workspace(0, sizeof(uint32_t) * scan_sums_words(nb), &rc);
     if (rc) return rc;
-    hipLaunchKernelGGL(compact_flags_kernel, dim3(grid_for(nb)), dim3(256), 0, stream(), flags, rank, nb, d_out);
-    MSBG_HIP(hipGetLastError());
-    MSBG_HIP(hipStreamSynchronize(stream()));
-    *n_out = total;
-    return MSBG_OK;
+    return active_blocks_into(d_pos, n, r_scan, nx, ny, nz, flags, flags + nb, sums, d_out, n_out);
 }
 
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst) {

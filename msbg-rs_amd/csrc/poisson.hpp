@@ -122,7 +122,7 @@ struct msbg_poisson {
     hipGraphExec_t exec = nullptr;
     int g_mode = -1, g_storage = -1, g_sweeps = -1;
     float g_omega = 0.0f;
-    const void *g_x = nullptr, *g_b = nullptr;
+    const void *g_x = nullptr, *g_lo = nullptr, *g_b = nullptr;
     // graph of a run of Jacobi sweeps, keyed by its arguments
     hipGraphExec_t jexec = nullptr;
     int j_storage = -1, j_sweeps = -1;

@@ -109,15 +109,25 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply(const uint32_t *in, u
     }
 }
 
+int64_t scan_sums_words(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total_host) {
     if (n <= 0) {
         if (total_host) *total_host = 0;
         return MSBG_OK;
     }
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     int rc;
-    uint32_t *sums = (uint32_t *)workspace(0, sizeof(uint32_t) * (tiles + 1), &rc);
+    uint32_t *sums = (uint32_t *)workspace(0, sizeof(uint32_t) * scan_sums_words(n), &rc);
     if (rc) return rc;
+    return scan_exclusive_u32_with(in, out, n, sums, total_host);
+}
+
+int scan_exclusive_u32_with(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *sums, uint32_t *total_host) {
+    if (n <= 0) {
+        if (total_host) *total_host = 0;
+        return MSBG_OK;
+    }
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(scan_tile_sums, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream(), in, n, sums);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, stream(), sums, tiles, sums + tiles);
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)tiles), dim3(kScanThreads), 0, stream(), in, out, n, sums);

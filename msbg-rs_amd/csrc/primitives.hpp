@@ -14,6 +14,10 @@ void *workspace(int id, size_t bytes, int *rc);
 // out[i] = sum_{j<i} in[i]; returns the total in *total_host if non-null
 // (synchronizes only then).  in and out may alias.
 int scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total_host);
+// The same scan with the caller's tile-sum buffer of scan_sums_words(n) words
+// (no workspace allocation).
+int scan_exclusive_u32_with(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *sums, uint32_t *total_host);
+int64_t scan_sums_words(int64_t n);
 
 // Stable sort of (keys, vals) by the low `key_bits` bits of keys, in place.
 int radix_sort_pairs(uint32_t *keys, uint32_t *vals, int64_t n, int key_bits);

@@ -156,7 +156,7 @@ int msbg_grid_new(int64_t sx, int64_t sy, int64_t sz, int log2_bsx, float empty_
     MSBG_CHECK(out, MSBG_EINVAL, "msbg_grid_new: null out");
     *out = nullptr;
     MSBG_CHECK(sx > 0 && sy > 0 && sz > 0, MSBG_EINVAL, "msbg_grid_new: domain must be non-empty");
-    MSBG_CHECK(log2_bsx >= 1 && log2_bsx <= 4, MSBG_EINVAL, "msbg_grid_new: block edge must be 2..16");
+    MSBG_CHECK(log2_bsx >= 0 && log2_bsx <= 4, MSBG_EINVAL, "msbg_grid_new: block edge must be 1..16");
     MSBG_CHECK(capacity_blocks >= 0, MSBG_EINVAL, "msbg_grid_new: negative capacity");
     MSBG_CHECK(sx < (1 << 30) && sy < (1 << 30) && sz < (1 << 30), MSBG_EINVAL, "msbg_grid_new: domain too large");
     msbg_grid *g = new msbg_grid();

@@ -383,6 +383,150 @@ __global__ __launch_bounds__(kThreads) void ir_update_kernel(const PBlock *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Compensated iterate: x carried as the unevaluated sum hi + lo of two FP32
+// fields (|lo| <= ulp(hi) / 2), so the refinement can take the residual below
+// the rounding floor of one FP32 field (~2.3e-7 ||x|| / ||b||).  Error-free
+// transformations (Knuth TwoSum, Dekker FastTwoSum, fma TwoProd) in FP32.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float two_sum(float a, float b, float &err) {
+    const float s = a + b;
+    const float bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// (hi, lo) + v, renormalised.
+__device__ __forceinline__ void pair_add(float &hi, float &lo, float v) {
+    float t;
+    const float h = two_sum(hi, v, t);
+    float u;
+    hi = two_sum(h, lo + t, u);
+    lo = u;
+}
+
+// r = b - A (hi + lo) of column cell (i, j), rounded once to FP32: the hi part
+// of A x (6 hi_c and the six hi neighbours) and b are summed error-free, the
+// lo part (|A lo| ~ 2^-22 |x|) in plain FP32.
+template <int Z>
+__device__ __forceinline__ float col_residual_pair(const Col<MSBG_STORAGE_F32, Z> &Q,
+                                                   const Col<MSBG_STORAGE_F32, Z> &QL, const Dom &d, int i, int j,
+                                                   float bi) {
+    constexpr int W = 4;
+    auto taps = [&](const Col<MSBG_STORAGE_F32, Z> &C, float *t) {
+        const float xc = C.c[j + 1].at(i);
+        t[0] = xc;
+        t[1] = i == 0 ? C.l[j] : C.c[j + 1].at(i - 1);
+        t[2] = Q.gx0 + i + 1 >= d.sx ? xc : (i == W - 1 ? C.r[j] : C.c[j + 1].at(i + 1));
+        t[3] = C.ym[j].at(i);
+        t[4] = C.yp[j].at(i);
+        t[5] = C.c[j].at(i);
+        t[6] = Q.gz0 + j + 1 >= d.sz ? xc : C.c[j + 2].at(i);
+    };
+    float h[7], l[7];
+    taps(Q, h);
+    taps(QL, l);
+    const float p6 = 6.0f * h[0];
+    float err = __builtin_fmaf(6.0f, h[0], -p6);  // 6 hi_c = p6 + err exactly
+    float e, acc = two_sum(bi, -p6, e);
+    e -= err;
+#pragma unroll
+    for (int k = 1; k < 7; ++k) {
+        float t;
+        acc = two_sum(acc, h[k], t);
+        e += t;
+    }
+    const float slo = ((((l[1] + l[2]) + l[3]) + l[4]) + l[5]) + l[6];
+    e += __builtin_fmaf(-6.0f, l[0], slo);
+    return acc + e;
+}
+
+// Residual of a compensated FP32 iterate, with the FP64 partial norms of
+// residual_kernel (same reduction order).
+template <int Z>
+__global__ __launch_bounds__(kThreads) void residual_pair_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                                 const float *x, const float *xlo, const float *b,
+                                                                 float *r, double2 *partial) {
+    constexpr int W = 4;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / ColMap<MSBG_STORAGE_F32, Z>::KS];
+    Col<MSBG_STORAGE_F32, Z> Q, QL;
+    load_col<MSBG_STORAGE_F32, Z>(B, d, x, b, wg, Q);
+    load_col<MSBG_STORAGE_F32, Z>(B, d, xlo, xlo, wg, QL);
+    double rr = 0.0, bb = 0.0;
+    if (Q.n) {
+#pragma unroll
+        for (int j = 0; j < Z; ++j) {
+            if (j >= Q.nz) break;
+            float t[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                t[i] = 0.0f;
+                if (i < Q.n) {
+                    const float bi = Q.b[j].at(i);
+                    t[i] = col_residual_pair<Z>(Q, QL, d, i, j, bi);
+                    rr += (double)t[i] * (double)t[i];
+                    bb += (double)bi * (double)bi;
+                }
+            }
+            if (r) {
+                const i64 cj = Q.c0 + 256 * j;
+                if (Q.n == W) {
+                    *reinterpret_cast<float4 *>(r + cj) = make_float4(t[0], t[1], t[2], t[3]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < W; ++i)
+                        if (i < Q.n) r[cj + i] = t[i];
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rr += __shfl_xor(rr, o);
+        bb += __shfl_xor(bb, o);
+    }
+    __shared__ double s_rr[kThreads / 64], s_bb[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s_rr[threadIdx.x >> 6] = rr;
+        s_bb[threadIdx.x >> 6] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+            a += s_rr[w];
+            c += s_bb[w];
+        }
+        partial[blockIdx.x] = make_double2(a, c);
+    }
+}
+
+// Refinement step end on a compensated iterate: (x, lo) += (scale / extra) e
+// (the product is exact: both factors of the scale are powers of two).
+template <int ST, int Z>
+__global__ __launch_bounds__(kThreads) void ir_update_pair_kernel(const PBlock *__restrict__ blocks, Dom d,
+                                                                  const typename Sto<ST>::T *e, const float *scale,
+                                                                  float extra, float *x, float *xlo) {
+    constexpr int W = ColMap<ST, Z>::W;
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const PBlock B = blocks[wg / ColMap<ST, Z>::KS];
+    Col<ST, Z> Q;
+    int vid;
+    if (!col_origin<ST, Z>(B, d, wg, Q, vid)) return;
+    const float s = scale[0] / extra;
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        if (j >= Q.nz) break;
+        const i64 cj = Q.c0 + 256 * j;
+        for (int i = 0; i < Q.n; ++i) {
+            float hi = x[cj + i], lo = xlo[cj + i];
+            pair_add(hi, lo, s * Sto<ST>::ld(e, cj + i));
+            x[cj + i] = hi;
+            xlo[cj + i] = lo;
+        }
+    }
+}
+
 __global__ void zero_u32_kernel(uint32_t *p, i64 n) {
     for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) p[i] = 0u;
 }
@@ -561,6 +705,42 @@ void ir_update(msbg_poisson *p, hipStream_t s, int st, float *x, float extra) {
         ir_update_st<MSBG_STORAGE_BF16>(p, s, x, extra);
 }
 
+// r = b - A (x + lo) and its norms (compensated iterate; FP32 fields).
+void residual_pair(msbg_poisson *p, hipStream_t s, const float *x, const float *lo, const float *b, float *r) {
+    constexpr int Z = 1;  // two columns of loads per thread: the shallowest column
+    const int n = col_groups<MSBG_STORAGE_F32, Z>(p);
+    residual_pair_kernel<Z><<<n, kThreads, 0, s>>>(p->d_blocks, p->dom, x, lo, b, r, p->d_partial);
+    reduce_norms_kernel<<<1, 256, 0, s>>>(p->d_partial, n, (double)p->n_cells, p->d_norm, p->d_scale);
+}
+
+template <int ST>
+void ir_update_pair_st(msbg_poisson *p, hipStream_t s, float *x, float *lo, float extra) {
+    using T = typename Sto<ST>::T;
+    with_z<ST>(p, [&](auto zc) {
+        constexpr int Z = decltype(zc)::value;
+        ir_update_pair_kernel<ST, Z><<<col_groups<ST, Z>(p), kThreads, 0, s>>>(p->d_blocks, p->dom,
+                                                                                (const T *)p->d_e, p->d_scale,
+                                                                                extra, x, lo);
+    });
+}
+
+void ir_update_pair(msbg_poisson *p, hipStream_t s, int st, float *x, float *lo, float extra) {
+    if (st == MSBG_STORAGE_F32)
+        ir_update_pair_st<MSBG_STORAGE_F32>(p, s, x, lo, extra);
+    else if (st == MSBG_STORAGE_F16)
+        ir_update_pair_st<MSBG_STORAGE_F16>(p, s, x, lo, extra);
+    else
+        ir_update_pair_st<MSBG_STORAGE_BF16>(p, s, x, lo, extra);
+}
+
+// Residual of the iterate: plain FP32 x, or the compensated x + lo.
+void residual_of(msbg_poisson *p, hipStream_t s, const float *x, const float *lo, const float *b, float *r) {
+    if (lo)
+        residual_pair(p, s, x, lo, b, r);
+    else
+        residual(p, s, MSBG_STORAGE_F32, x, b, r);
+}
+
 // Inner fields for `storage`; slots outside the listed blocks stay zero (the
 // error equation has homogeneous boundary values).
 int ensure_inner(msbg_poisson *p, int storage) {
@@ -593,7 +773,7 @@ int ensure_inner(msbg_poisson *p, int storage) {
 // on e in `storage`, x += s e, then the residual (into r).  mode 2: the same
 // refinement step with one multigrid V-cycle (`sweeps` pre- and
 // post-smoothing sweeps per level) as the inner solve.
-void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, const float *b, float omega,
+void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, float *lo, const float *b, float omega,
             int sweeps) {
     if (mode == 0) {
         rb_sweeps(p, s, MSBG_STORAGE_F32, x, b, omega, sweeps);
@@ -611,22 +791,26 @@ void period(msbg_poisson *p, hipStream_t s, int mode, int storage, float *x, con
             mg::vcycle(p, s, storage, omega, sweeps, fold);
         else
             pband::rb_sweeps(p, s, storage, p->d_e, p->d_rt, omega, sweeps, false, fold);
-        ir_update(p, s, storage, x, extra);
-        residual(p, s, MSBG_STORAGE_F32, x, b, p->d_r);
+        if (lo)
+            ir_update_pair(p, s, storage, x, lo, extra);
+        else
+            ir_update(p, s, storage, x, extra);
+        residual_of(p, s, x, lo, b, p->d_r);
     }
 }
 
 // Bands from this many cells up run their V-cycles as direct launches.
 constexpr int64_t kDirectCycleCells = (int64_t)64 << 20;
 
-int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const float *b, float omega, int sweeps) {
+int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, float *lo, const float *b, float omega,
+                     int sweeps) {
     if (!(p->exec && p->g_mode == mode && p->g_storage == storage && p->g_sweeps == sweeps && p->g_omega == omega &&
-          p->g_x == x && p->g_b == b)) {
+          p->g_x == x && p->g_lo == lo && p->g_b == b)) {
         drop_graph(p);
         if (!p->g_done) MSBG_HIP(hipEventCreateWithFlags(&p->g_done, hipEventDisableTiming));
         if (!p->cap_stream) MSBG_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         MSBG_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
-        period(p, p->cap_stream, mode, storage, x, b, omega, sweeps);
+        period(p, p->cap_stream, mode, storage, x, lo, b, omega, sweeps);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(p->cap_stream, &graph);
         MSBG_HIP(e);
@@ -639,6 +823,7 @@ int run_period_graph(msbg_poisson *p, int mode, int storage, float *x, const flo
         p->g_sweeps = sweeps;
         p->g_omega = omega;
         p->g_x = x;
+        p->g_lo = lo;
         p->g_b = b;
     }
     MSBG_HIP(hipGraphLaunch(p->exec, stream()));
@@ -864,6 +1049,18 @@ int msbg_poisson_residual(msbg_poisson *p, int storage, const void *x, const voi
     return read_norms(p, r2, b2);
 }
 
+int msbg_poisson_residual_pair(msbg_poisson *p, const float *x, const float *x_lo, const float *b, float *r,
+                               double *r2, double *b2) {
+    MSBG_CHECK(p && x && x_lo && b && r2 && b2, MSBG_EINVAL, "null argument");
+    if (!p->n_active) {
+        *r2 = *b2 = 0.0;
+        return MSBG_OK;
+    }
+    residual_pair(p, stream(), x, x_lo, b, r);
+    MSBG_HIP(hipGetLastError());
+    return read_norms(p, r2, b2);
+}
+
 int msbg_poisson_convert(int src_storage, const void *src, int dst_storage, void *dst, int64_t n_elems) {
     MSBG_CHECK(src && dst && n_elems >= 0, MSBG_EINVAL, "bad argument");
     int rc = check_storage(src_storage);
@@ -895,12 +1092,17 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
     MSBG_CHECK(prm->omega > 0.0f && prm->omega < 2.0f, MSBG_EINVAL, "omega must be in (0, 2)");
     MSBG_CHECK(prm->sweeps_per_check >= 1 && prm->max_sweeps >= 0, MSBG_EINVAL, "bad sweep counts");
     MSBG_CHECK(prm->tol >= 0.0, MSBG_EINVAL, "negative tolerance");
+    MSBG_CHECK((prm->flags & ~MSBG_SOLVE_DIRECT) == 0, MSBG_EINVAL, "unknown solve flags");
     *st = msbg_poisson_stats{};
     if (!p->n_active) {
         st->converged = 1;
         return MSBG_OK;
     }
     MSBG_CHECK(!prm->learned || prm->phi, MSBG_EINVAL, "the learned correction needs the phase field grid");
+    float *lo = prm->x_lo;
+    MSBG_CHECK(!lo || prm->refine >= 1, MSBG_EINVAL, "a compensated iterate (x_lo) needs refine = 1 or 2");
+    MSBG_CHECK(!lo || !prm->learned, MSBG_EINVAL, "the learned correction takes a plain FP32 iterate (x_lo = NULL)");
+    MSBG_CHECK(!lo || lo != x, MSBG_EINVAL, "x_lo must be a field of its own");
     if (prm->phi)
         MSBG_CHECK(prm->phi->L == 4 && prm->phi->sx == p->dom.sx && prm->phi->sy == p->dom.sy &&
                        prm->phi->sz == p->dom.sz,
@@ -921,7 +1123,7 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
     MSBG_HIP(hipEventCreate(&e0));
     MSBG_HIP(hipEventCreate(&e1));
     MSBG_HIP(hipEventRecord(e0, stream()));
-    residual(p, stream(), MSBG_STORAGE_F32, x, b, p->d_r);
+    residual_of(p, stream(), x, lo, b, p->d_r);
     MSBG_HIP(hipGetLastError());
     double r2 = 0.0, b2 = 0.0;
     rc = read_norms(p, &r2, &b2);
@@ -961,15 +1163,15 @@ int msbg_poisson_solve(msbg_poisson *p, const msbg_poisson_params *prm, float *x
         // fine-level red-black sweeps of one check period
         const int per = mode == 2 ? 2 * prm->sweeps_per_check : prm->sweeps_per_check;
         if (!std::isfinite(rel) || st->sweeps + per > prm->max_sweeps) break;
-        if (mode == 2 && p->n_cells >= kDirectCycleCells) {
+        if ((prm->flags & MSBG_SOLVE_DIRECT) || (mode == 2 && p->n_cells >= kDirectCycleCells)) {
             // a cycle of a band this large is milliseconds of GPU work, which
             // hides its ~60 launches; a graph would cost its instantiation
             // (~4 ms) on every new band
-            period(p, stream(), mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+            period(p, stream(), mode, prm->storage, x, lo, b, prm->omega, prm->sweeps_per_check);
             hipError_t le = hipGetLastError();
             if (le != hipSuccess) rc = hip_fail(le, "msbg_poisson_solve");
         } else {
-            rc = run_period_graph(p, mode, prm->storage, x, b, prm->omega, prm->sweeps_per_check);
+            rc = run_period_graph(p, mode, prm->storage, x, lo, b, prm->omega, prm->sweeps_per_check);
         }
         if (rc) break;
         st->sweeps += per;

@@ -1149,8 +1149,12 @@ __global__ __launch_bounds__(256) void divergence_kernel(GridView u, GridView v,
 
 // Pressure projection of the face velocities: face a of cell c takes
 // u_a(c) - scale (p(c + e_a) - p(c)), the neighbour outside the domain being
-// the cell itself; one thread per cell, the three faces it owns.
+// the cell itself; one thread per cell, the three faces it owns.  LO: p is
+// the compensated pair p + plo (plo over p's slot layout), and the gradient
+// is (p(c + e_a) - p(c)) + (plo(c + e_a) - plo(c)).
+template <bool LO>
 __global__ __launch_bounds__(256) void subtract_gradient_kernel(GridView u, GridView v, GridView w, GridView p,
+                                                                const float *__restrict__ plo,
                                                                 const int32_t *__restrict__ bids, int64_t n,
                                                                 float scale) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1163,13 +1167,23 @@ __global__ __launch_bounds__(256) void subtract_gradient_kernel(GridView u, Grid
     const float pc = voxel_at(p, x, y, z);
     const float pn[3] = {x + 1 < p.sx ? voxel_at(p, x + 1, y, z) : pc, y + 1 < p.sy ? voxel_at(p, x, y + 1, z) : pc,
                          z + 1 < p.sz ? voxel_at(p, x, y, z + 1) : pc};
+    float lc = 0.0f, ln[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (LO) {
+        GridView q = p;
+        q.slab = const_cast<float *>(plo);
+        lc = voxel_at(q, x, y, z);
+        ln[0] = x + 1 < p.sx ? voxel_at(q, x + 1, y, z) : lc;
+        ln[1] = y + 1 < p.sy ? voxel_at(q, x, y + 1, z) : lc;
+        ln[2] = z + 1 < p.sz ? voxel_at(q, x, y, z + 1) : lc;
+    }
     const GridView ch[3] = {u, v, w};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int slot = ch[a].b2s[bid];
         if (slot < 2) continue;
         float *f = ch[a].slab + ((int64_t)slot << 12) + vid;
-        const float g = pn[a] - pc;
+        float g = pn[a] - pc;
+        if constexpr (LO) g = g + (ln[a] - lc);
         const float t = scale * g;
         *f = *f - t;
     }
@@ -1178,8 +1192,11 @@ __global__ __launch_bounds__(256) void subtract_gradient_kernel(GridView u, Grid
 }  // namespace
 }  // namespace msbg
 
-extern "C" int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
-                                          const int64_t *active, int64_t n, float scale) {
+namespace msbg {
+namespace {
+
+int subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p, const float *d_plo,
+                      const int64_t *active, int64_t n, float scale) {
     MSBG_CHECK(vel && vel[0] && vel[1] && vel[2] && p_layout && d_p, MSBG_EINVAL, "subtract gradient: null argument");
     MSBG_CHECK(n >= 0 && (n == 0 || active), MSBG_EINVAL, "subtract gradient: bad active list");
     for (int a = 0; a < 3; ++a)
@@ -1197,11 +1214,30 @@ extern "C" int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid
     MSBG_HIP(hipMemcpyAsync(d_bids, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, stream()));
     GridView pv = p_layout->view();
     pv.slab = const_cast<float *>(d_p);
-    subtract_gradient_kernel<<<(int)((n * 4096 + 255) / 256), 256, 0, stream()>>>(vel[0]->view(), vel[1]->view(),
-                                                                                  vel[2]->view(), pv, d_bids, n, scale);
+    const int grid = (int)((n * 4096 + 255) / 256);
+    if (d_plo)
+        subtract_gradient_kernel<true><<<grid, 256, 0, stream()>>>(vel[0]->view(), vel[1]->view(), vel[2]->view(), pv,
+                                                                    d_plo, d_bids, n, scale);
+    else
+        subtract_gradient_kernel<false><<<grid, 256, 0, stream()>>>(vel[0]->view(), vel[1]->view(), vel[2]->view(),
+                                                                     pv, nullptr, d_bids, n, scale);
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;
+}
+
+}  // namespace
+}  // namespace msbg
+
+extern "C" int msbg_mac_subtract_gradient(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
+                                          const int64_t *active, int64_t n, float scale) {
+    return msbg::subtract_gradient(vel, p_layout, d_p, nullptr, active, n, scale);
+}
+
+extern "C" int msbg_mac_subtract_gradient_pair(msbg_grid *const *vel, const msbg_grid *p_layout, const float *d_p,
+                                               const float *d_p_lo, const int64_t *active, int64_t n, float scale) {
+    MSBG_CHECK(d_p_lo, MSBG_EINVAL, "subtract gradient: null low part");
+    return msbg::subtract_gradient(vel, p_layout, d_p, d_p_lo, active, n, scale);
 }
 
 extern "C" int msbg_mac_divergence(msbg_grid *const *vel, msbg_grid *dst, const int64_t *active, int64_t n) {

@@ -40,9 +40,13 @@ class FlipStep:
         self.capacity_blocks = capacity_blocks
 
     def run(self, pos, vel, mass, r_p=2.0, alpha=0.95, storage=Storage.F32, refine=False, omega=1.9,
-            sweeps_per_check=4, max_sweeps=20000, tol=1e-6, learned=None, phi=None, multigrid=False):
+            sweeps_per_check=4, max_sweeps=20000, tol=1e-6, learned=None, phi=None, multigrid=False,
+            compensated=False):
         """One step.  pos / vel / mass: device arrays (n x 3, n x 3, n); the
-        solve's arguments are PoissonBand.solve's.
+        solve's arguments are PoissonBand.solve's.  compensated (refine or
+        multigrid): the pressure is the compensated pair p + p_lo of two FP32
+        fields through the solve and the projection, so the solve can reach a
+        residual below one FP32 field's rounding floor.
         Returns (sorted positions, new velocities, stats) with stats holding
         the solve's msbg_poisson_stats, the active block count, the band's
         cell count and per-stage ms (HIP events)."""
@@ -72,10 +76,11 @@ class FlipStep:
         lib.call("msbg_memcpy_dtod", b.ptr, dev_slab, band.n_slots * 4096 * 4)
         rec(3)
         p = band.field(0)
+        p_lo = band.field(0) if compensated else None
         st = band.solve(p, b, storage=storage, refine=refine, omega=omega, sweeps_per_check=sweeps_per_check,
-                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi, multigrid=multigrid)
+                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi, multigrid=multigrid, x_lo=p_lo)
         rec(4)
-        self.mac.subtract_gradient(layout, p, active, -1.0)
+        self.mac.subtract_gradient(layout, p, active, -1.0, p_lo=p_lo)
         rec(5)
         out = g2p_flip(self.mac, self.mac_tilde, ps, vs, alpha)
         rec(6)
@@ -88,6 +93,7 @@ class FlipStep:
         stats = {"active_blocks": int(k), "band_blocks": int(band_blocks.size), "cells": band.n_cells, "solve": st,
                  "ms": stage_ms}
         self.last = (layout, band, p, b, active, band_blocks)
+        self.p_lo = p_lo
         self.sorted = (ps, vs, ms)
         return ps, out, stats
 

@@ -88,6 +88,7 @@ _SIGS = {
     "msbg_block_topology": [vp, i64, i64, i64, i32, vp, vp, vp, vp, C.POINTER(C.c_int64)],
     "msbg_mac_divergence": [vp, vp, vp, i64],
     "msbg_mac_subtract_gradient": [vp, vp, vp, vp, i64, f32],
+    "msbg_mac_subtract_gradient_pair": [vp, vp, vp, vp, vp, i64, f32],
     "msbg_poisson_new": [vp, vp, i64, PP],
     "msbg_poisson_free": [vp],
     "msbg_poisson_info": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
@@ -95,6 +96,7 @@ _SIGS = {
     "msbg_poisson_rbgs": [vp, i32, vp, vp, i32, f32],
     "msbg_poisson_jacobi_sweeps": [vp, i32, vp, vp, vp, f32, i32],
     "msbg_poisson_residual": [vp, i32, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "msbg_poisson_residual_pair": [vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     "msbg_poisson_convert": [i32, vp, i32, vp, i64],
     "msbg_poisson_solve": [vp, vp, vp, vp, vp],
     "msbg_learned_param_count": [C.POINTER(C.c_int)],
@@ -116,8 +118,8 @@ _RESTYPE = {"msbg_last_error": C.c_char_p}
 class PoissonParams(C.Structure):
     """msbg_poisson_params (include/msbg_hip.h)."""
     _fields_ = [("storage", C.c_int), ("refine", C.c_int), ("omega", C.c_float), ("sweeps_per_check", C.c_int),
-                ("max_sweeps", C.c_int), ("reserved", C.c_int), ("tol", C.c_double), ("learned", C.c_void_p),
-                ("phi", C.c_void_p)]
+                ("max_sweeps", C.c_int), ("flags", C.c_int), ("tol", C.c_double), ("learned", C.c_void_p),
+                ("phi", C.c_void_p), ("x_lo", C.c_void_p)]
 
 
 class PoissonStats(C.Structure):

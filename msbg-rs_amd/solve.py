@@ -89,8 +89,16 @@ class PoissonBand:
                       C.byref(r2), C.byref(b2))
         return r2.value, b2.value
 
+    def residual_pair(self, x, x_lo, b, r=None):
+        """(||b - A (x + x_lo)||^2, ||b||^2) of a compensated FP32 iterate; r
+        (FP32 field) receives the residual, rounded once."""
+        r2, b2 = C.c_double(), C.c_double()
+        self.lib.call("msbg_poisson_residual_pair", self.h, x.ptr, x_lo.ptr, b.ptr, None if r is None else r.ptr,
+                      C.byref(r2), C.byref(b2))
+        return r2.value, b2.value
+
     def solve(self, x, b, storage=Storage.F32, refine=False, omega=1.9, sweeps_per_check=10, max_sweeps=10000,
-              tol=1e-6, learned=None, phi=None, multigrid=False):
+              tol=1e-6, learned=None, phi=None, multigrid=False, x_lo=None, direct=False):
         """Red-black SOR solve of A x = b (x, b FP32 fields; x = initial guess,
         updated in place).  refine=True: FP32 iterative refinement with the
         inner sweeps in `storage`.  multigrid=True: FP32 iterative refinement
@@ -100,10 +108,15 @@ class PoissonBand:
         `storage`; max_sweeps caps the band's sweeps (2 sweeps_per_check per
         cycle).  learned (a CorrectorNet) with phi (the phase-field
         DeviceGrid): a correction is tried after every check period.  Returns
-        the msbg_poisson_stats fields."""
+        the msbg_poisson_stats fields.  x_lo (an FP32 field, refine or multigrid
+        only): the low part of a compensated iterate x + x_lo, carried through
+        the refinement with error-free FP32 sums (x_lo zero on entry for a
+        fresh solve).  direct: every check period's kernels are launched
+        directly instead of replayed as a captured graph (same results)."""
         mode = 2 if multigrid else int(refine)
-        prm = _lib.PoissonParams(storage, mode, omega, sweeps_per_check, max_sweeps, 0, tol,
-                                 learned.h if learned is not None else None, phi.h if phi is not None else None)
+        prm = _lib.PoissonParams(storage, mode, omega, sweeps_per_check, max_sweeps, 1 if direct else 0, tol,
+                                 learned.h if learned is not None else None, phi.h if phi is not None else None,
+                                 x_lo.ptr if x_lo is not None else None)
         st = _lib.PoissonStats()
         self.lib.call("msbg_poisson_solve", self.h, C.byref(prm), x.ptr, b.ptr, C.byref(st))
         return {k: getattr(st, k) for k, _ in _lib.PoissonStats._fields_}

@@ -99,6 +99,74 @@ def make_pairs(pkg, lib, seeds, states, every, out_scale, log):
     return X, T, total
 
 
+def make_pairs_mg(pkg, lib, seeds, rhs_per_scene, cycles, out_scale, log):
+    """Pairs from multigrid trajectories: for each scene and right-hand side,
+    the iterate after each of `cycles` V(2, 2)-cycles (FP32 refinement, FP16
+    band storage) from x = 0 gives (residual / 2^k + phi, (x* - x) / 2^k /
+    out_scale) per block."""
+    from msbg_rs_amd import scenes
+    xs_, ts_ = [], []
+    total = 0
+    for s in seeds:
+        sc = scenes.noisy_sphere_scene(S=128, n_levels=2, band=8.0, seed=s, amp=6.0)
+        g_h, bids = sc.grids[0], sc.level_blocks(0)
+        phi = pkg.DeviceGrid.from_host(g_h, lib=lib)
+        band = pkg.PoissonBand(phi, bids, lib=lib)
+        for j in range(rhs_per_scene):
+            b = lib.to_device(scenes.poisson_rhs(g_h, bids, seed=2000 + 97 * s + j))
+            xs = band.field(0)
+            band.solve(xs, b, storage=0, refine=True, omega=1.9, sweeps_per_check=10, max_sweeps=20000, tol=2e-7)
+            x, r = band.field(0), band.field(0)
+            n = band.n_active
+            X = lib.empty((cycles * n, 20, 20, 20, 2), np.uint16)
+            T = lib.empty((cycles * n, 4096), np.float32)
+            for k in range(cycles):
+                band.solve(x, b, storage=1, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=4, tol=0.0)
+                r2, _ = band.residual(0, x, b, r)
+                inv = pow2_inv(r2, band.n_cells)
+                x0 = band.net_input(phi, r, inv)
+                lib.call("msbg_memcpy_dtod", X.ptr + k * n * 32000, x0.ptr, n * 32000)
+                t = band.error_target(x, xs, inv / out_scale)
+                lib.call("msbg_memcpy_dtod", T.ptr + k * n * 16384, t.ptr, n * 16384)
+            total += cycles * n
+            xs_.append(X)
+            ts_.append(T)
+        log("scene %d: %d pairs so far" % (s, total))
+    X = lib.empty((total, 20, 20, 20, 2), np.uint16)
+    T = lib.empty((total, 4096), np.float32)
+    off = 0
+    for cx, ct in zip(xs_, ts_):
+        m = cx.shape[0]
+        lib.call("msbg_memcpy_dtod", X.ptr + off * 32000, cx.ptr, m * 32000)
+        lib.call("msbg_memcpy_dtod", T.ptr + off * 16384, ct.ptr, m * 16384)
+        off += m
+        cx.free()
+        ct.free()
+    return X, T, total
+
+
+def evaluate_mg(pkg, lib, net, tol=1e-6):
+    """V-cycles and wall time to tol on the 128^3 shell, multigrid (FP16 band
+    storage) with and without a learned correction after every cycle."""
+    from msbg_rs_amd import scenes
+    g, bids, _ = scenes.shell_scene()
+    phi = pkg.DeviceGrid.from_host(g, lib=lib)
+    band = pkg.PoissonBand(phi, bids, lib=lib)
+    b = lib.to_device(scenes.poisson_rhs(g, bids))
+    out = {}
+    for learned in (False, True):
+        best = None
+        for _ in range(3):
+            x = band.field(0)
+            st = band.solve(x, b, storage=1, multigrid=True, omega=1.0, sweeps_per_check=2, max_sweeps=400, tol=tol,
+                            learned=net if learned else None, phi=phi if learned else None)
+            if best is None or st["ms"] < best["ms"]:
+                best = st
+        out["f16_mg" + ("_learned" if learned else "")] = {
+            k: (float(v) if isinstance(v, float) else v) for k, v in best.items()}
+    return out
+
+
 def evaluate(pkg, lib, net, tol=1e-6, every=4):
     """Sweeps and wall time to tol on the 128^3 shell (not in the training
     scenes): plain vs learned, FP32 and FP16 / BF16 refinement."""
@@ -134,6 +202,11 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out-scale", type=float, default=64.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--solver", choices=("sor", "mg"), default="sor",
+                    help="trajectories the pairs come from: red-black SOR every --every sweeps, or multigrid "
+                         "V-cycles (--rhs right-hand sides per scene, --cycles iterates each)")
+    ap.add_argument("--rhs", type=int, default=8)
+    ap.add_argument("--cycles", type=int, default=6)
     args = ap.parse_args()
 
     def log(*a):
@@ -143,7 +216,10 @@ def main():
     lib = pkg.lib.load()
     from msbg_rs_amd.learned import init_params
     t0 = time.time()
-    X, T, n = make_pairs(pkg, lib, range(100, 100 + args.scenes), args.states, args.every, args.out_scale, log)
+    if args.solver == "mg":
+        X, T, n = make_pairs_mg(pkg, lib, range(100, 100 + args.scenes), args.rhs, args.cycles, args.out_scale, log)
+    else:
+        X, T, n = make_pairs(pkg, lib, range(100, 100 + args.scenes), args.states, args.every, args.out_scale, log)
     lib.synchronize()
     t_gen = time.time() - t0
     log("pairs: %d in %.1fs" % (n, t_gen))
@@ -158,7 +234,7 @@ def main():
     t_train = time.time() - t0
     losses = np.concatenate(losses)
     net.save(args.out)
-    ev = evaluate(pkg, lib, net)
+    ev = evaluate_mg(pkg, lib, net) if args.solver == "mg" else evaluate(pkg, lib, net)
     print(json.dumps({"pairs": n, "gen_s": t_gen, "train_s": t_train, "steps": int(losses.size),
                       "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-20:].mean()),
                       "eval": ev}), flush=True)

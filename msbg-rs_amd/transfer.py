@@ -63,12 +63,17 @@ class MacGrid:
         a = np.ascontiguousarray(active, np.int64)
         self.lib.call("msbg_mac_divergence", _handles(self.velocity), dst.h, a.ctypes.data, a.size)
 
-    def subtract_gradient(self, p_layout: DeviceGrid, p, active, scale=1.0):
+    def subtract_gradient(self, p_layout: DeviceGrid, p, active, scale=1.0, p_lo=None):
         """Face velocities of the `active` blocks minus scale * grad p (p a
-        field over p_layout's slots, e.g. a PoissonBand solution)."""
+        field over p_layout's slots, e.g. a PoissonBand solution; p_lo: the low
+        part of a compensated solution p + p_lo)."""
         a = np.ascontiguousarray(active, np.int64)
-        self.lib.call("msbg_mac_subtract_gradient", _handles(self.velocity), p_layout.h, p.ptr, a.ctypes.data,
-                      a.size, scale)
+        if p_lo is None:
+            self.lib.call("msbg_mac_subtract_gradient", _handles(self.velocity), p_layout.h, p.ptr, a.ctypes.data,
+                          a.size, scale)
+        else:
+            self.lib.call("msbg_mac_subtract_gradient_pair", _handles(self.velocity), p_layout.h, p.ptr, p_lo.ptr,
+                          a.ctypes.data, a.size, scale)
 
 
 def particles_to_grid(mac: MacGrid, pos, vel, mass, r_p) -> int:

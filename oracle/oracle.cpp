@@ -584,6 +584,11 @@ void orc_sweep_density8(i64 sx, i64 sy, i64 sz, uint8_t *slab8, i64 n_slots, con
 // and level l+1.  The reference runs fill_multires only from benches
 // (multires_bench.rs:159-169); this is the sweep those benches time, made a
 // call.
+// FTZ / DAZ inside orc_sweep_multires' worker threads (the benches' pool
+// setting, thread_pool.rs:5-11); off by default, as the reference's tests run.
+static int g_multires_ftz = 0;
+void orc_set_multires_ftz(int on) { g_multires_ftz = on; }
+
 void orc_sweep_multires(i64 sx, i64 sy, i64 sz, int L, float *fine_slab, const i32 *fine_b2s,
                         const float *coarse_slab, const i32 *coarse_b2s, const uint8_t *levels, const i64 *active,
                         i64 n, int stencil, float dt, int iterations, int clamp01, int nthreads) {
@@ -602,6 +607,11 @@ void orc_sweep_multires(i64 sx, i64 sy, i64 sz, int L, float *fine_slab, const i
             const i64 cnt = counts[c];
 #pragma omp parallel num_threads(nthreads < 1 ? 1 : nthreads)
             {
+                const int ftz = g_multires_ftz;
+                if (ftz) {
+                    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+                    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+                }
                 std::vector<float> tile((size_t)HSX * HSX * HSX);
 #pragma omp for schedule(dynamic, 4)
                 for (i64 k = 0; k < cnt; ++k) {
@@ -617,6 +627,10 @@ void orc_sweep_multires(i64 sx, i64 sy, i64 sz, int L, float *fine_slab, const i
                         orc_kernel_meancurv(B, tile.data(), dt, clamp01, out);
                     else
                         orc_kernel_bilaplacian(B, tile.data(), dt, clamp01, out);
+                }
+                if (ftz) {
+                    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_OFF);
+                    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_OFF);
                 }
             }
             off += cnt;

@@ -299,6 +299,47 @@ void orc_poisson_residual(i64 sx, i64 sy, i64 sz, int storage, const void *x, co
     *b2 = bb;
 }
 
+// Residual of a compensated iterate x + xlo (FP32 fields, no reference
+// counterpart): b, -6 x_c (split exactly by an fma) and the six x neighbours
+// (x-, x+, y-, y+, z-, z+) summed with Knuth's TwoSum, their rounding errors
+// accumulated in FP32; the xlo part as fmaf(-6, xlo_c, s_lo) with s_lo summed
+// in the same neighbour order; r = sum + errors, rounded once.
+void orc_poisson_residual_pair(i64 sx, i64 sy, i64 sz, const float *x, const float *xlo, const float *b,
+                               const i32 *b2s, const i64 *active, i64 n, float *r, double *r2, double *b2) {
+    PGrid g(sx, sy, sz, b2s);
+    auto two_sum = [](float a, float c, float &err) {
+        const float s = a + c;
+        const float bb = s - a;
+        err = (a - (s - bb)) + (c - bb);
+        return s;
+    };
+    double rr = 0.0, bb = 0.0;
+    for_cells(g, active, n, [&](i64 i, i64 gx, i64 gy, i64 gz) {
+        const i64 q[7] = {g.index(gx, gy, gz),     g.index(gx - 1, gy, gz), g.index(gx + 1, gy, gz),
+                          g.index(gx, gy - 1, gz), g.index(gx, gy + 1, gz), g.index(gx, gy, gz - 1),
+                          g.index(gx, gy, gz + 1)};
+        const float bi = b[i];
+        const float p6 = 6.0f * x[q[0]];
+        const float err = fmaf(6.0f, x[q[0]], -p6);
+        float e;
+        float acc = two_sum(bi, -p6, e);
+        e -= err;
+        for (int k = 1; k < 7; ++k) {
+            float t;
+            acc = two_sum(acc, x[q[k]], t);
+            e += t;
+        }
+        const float slo = ((((xlo[q[1]] + xlo[q[2]]) + xlo[q[3]]) + xlo[q[4]]) + xlo[q[5]]) + xlo[q[6]];
+        e += fmaf(-6.0f, xlo[q[0]], slo);
+        const float t = acc + e;
+        if (r) r[i] = t;
+        rr += (double)t * (double)t;
+        bb += (double)bi * (double)bi;
+    });
+    *r2 = rr;
+    *b2 = bb;
+}
+
 // Weighted Jacobi: x_out = x + (omega/6) (b + s - 6 x) for every active cell.
 void orc_jacobi(i64 sx, i64 sy, i64 sz, int storage, const void *x_in, void *x_out, const void *b, const i32 *b2s,
                 const i64 *active, i64 n, float omega) {

@@ -626,9 +626,24 @@ extern "C" void orc_mac_divergence(i64 sx, i64 sy, i64 sz, const float *u, const
 // solve's residual.  Every cell of the listed blocks' clipped extent, in
 // place on each channel (the three channels' own bid -> slot tables).
 // ===========================================================================
+// The projection from a compensated pressure p + plo (plo over p's slot
+// layout, or NULL): the gradient is (p(c + e_a) - p(c)) + (plo(c + e_a) -
+// plo(c)).
+extern "C" void orc_mac_subtract_gradient_pair(i64 sx, i64 sy, i64 sz, float *u, const i32 *ub2s, float *v,
+                                               const i32 *vb2s, float *w, const i32 *wb2s, const float *p,
+                                               const float *plo, const i32 *pb2s, const i64 *active, i64 n,
+                                               float scale);
+
 extern "C" void orc_mac_subtract_gradient(i64 sx, i64 sy, i64 sz, float *u, const i32 *ub2s, float *v,
                                           const i32 *vb2s, float *w, const i32 *wb2s, const float *p,
                                           const i32 *pb2s, const i64 *active, i64 n, float scale) {
+    orc_mac_subtract_gradient_pair(sx, sy, sz, u, ub2s, v, vb2s, w, wb2s, p, nullptr, pb2s, active, n, scale);
+}
+
+extern "C" void orc_mac_subtract_gradient_pair(i64 sx, i64 sy, i64 sz, float *u, const i32 *ub2s, float *v,
+                                               const i32 *vb2s, float *w, const i32 *wb2s, const float *p,
+                                               const float *plo, const i32 *pb2s, const i64 *active, i64 n,
+                                               float scale) {
     const i64 nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nxy = nx * ny;
     auto idx = [&](const i32 *b2s, i64 x, i64 y, i64 z) {
         return (i64)b2s[(x >> 4) + (y >> 4) * nx + (z >> 4) * nxy] * 4096 + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8));
@@ -646,10 +661,18 @@ extern "C" void orc_mac_subtract_gradient(i64 sx, i64 sy, i64 sz, float *u, cons
                     const float pn[3] = {x + 1 < sx ? p[idx(pb2s, x + 1, y, z)] : pc,
                                          y + 1 < sy ? p[idx(pb2s, x, y + 1, z)] : pc,
                                          z + 1 < sz ? p[idx(pb2s, x, y, z + 1)] : pc};
+                    float lc = 0.0f, ln[3] = {0.0f, 0.0f, 0.0f};
+                    if (plo) {
+                        lc = plo[idx(pb2s, x, y, z)];
+                        ln[0] = x + 1 < sx ? plo[idx(pb2s, x + 1, y, z)] : lc;
+                        ln[1] = y + 1 < sy ? plo[idx(pb2s, x, y + 1, z)] : lc;
+                        ln[2] = z + 1 < sz ? plo[idx(pb2s, x, y, z + 1)] : lc;
+                    }
                     for (int a = 0; a < 3; ++a) {
                         if (cb[a][bid] < 2) continue;
                         float &f = ch[a][idx(cb[a], x, y, z)];
-                        const float g = pn[a] - pc;
+                        float g = pn[a] - pc;
+                        if (plo) g = g + (ln[a] - lc);
                         const float t = scale * g;
                         f = f - t;
                     }

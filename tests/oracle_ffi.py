@@ -38,6 +38,7 @@ def load():
     for k in ("laplacian", "meancurv", "bilaplacian"):
         _sig(lib, "orc_kernel_" + k, None, i32, f32p, f32, i32, f32p)
     _sig(lib, "orc_sweep", None, i64, i64, i64, f32p, i32p, i64p, i64, i32, f32, i32, i32, i32, i32)
+    _sig(lib, "orc_set_multires_ftz", None, i32)
     _sig(lib, "orc_sweep_multires", None, i64, i64, i64, i32, f32p, i32p, f32p, i32p, u8p, i64p, i64, i32, f32,
          i32, i32, i32)
     _sig(lib, "orc_halo_fill_level", None, i64p, C.c_void_p, C.c_void_p, u8p, i32, i32, i64, i32, f32p)
@@ -98,6 +99,10 @@ def load():
     _sig(lib, "orc_block_topology", i64, i64, i64, i64, u8p, i32, vp, u16p, i64p, i64p)
     _sig(lib, "orc_mac_subtract_gradient", None, i64, i64, i64, f32p, i32p, f32p, i32p, f32p, i32p, f32p, i32p, i64p,
          i64, f32)
+    _sig(lib, "orc_mac_subtract_gradient_pair", None, i64, i64, i64, f32p, i32p, f32p, i32p, f32p, i32p, f32p, f32p,
+         i32p, i64p, i64, f32)
+    _sig(lib, "orc_poisson_residual_pair", None, i64, i64, i64, vp, vp, vp, i32p, i64p, i64, vp,
+         C.POINTER(C.c_double), C.POINTER(C.c_double))
     _LIB = Oracle(lib)
     return _LIB
 

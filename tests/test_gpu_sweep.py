@@ -184,9 +184,9 @@ def test_multires_sweep_matches_oracle(pkg, orc, hip, stencil):
     assert_bitexact(df.to_host().slab, slab, "multires sweep")
 
 
-def level_pair(pkg, orc, hip, s, n_levels, seed):
+def level_pair(pkg, orc, hip, s, n_levels, seed, **kw):
     from test_oracle_multires import LevelGrid
-    lg = LevelGrid(pkg, orc, s, n_levels, seed=seed, lo=0.0, hi=1.0)
+    lg = LevelGrid(pkg, orc, s, n_levels, seed=seed, lo=0.0, hi=1.0, **kw)
     dev = [pkg.DeviceGrid.from_host(g, lib=hip) for g in lg.grids]
     return lg, dev, hip.to_device(lg.levels)
 
@@ -225,8 +225,10 @@ def test_level_sweep_graph_alternating(pkg, orc, hip):
 def test_level_sweep_five_levels(pkg, orc, hip):
     """Five levels: edge-1 blocks at level 4 take the radius-1 stencils; the
     radius-2 bi-Laplacian is refused with an error, and nothing is written."""
-    lg, dev, dl = level_pair(pkg, orc, hip, (160, 128, 128), 5, seed=9)
-    assert (lg.levels == 4).sum() > 10
+    # few fine blocks and few mixed ones, so that 2:1 regularisation leaves a
+    # staircase down to level 4 (41 edge-1 blocks here)
+    lg, dev, dl = level_pair(pkg, orc, hip, (160, 128, 128), 5, seed=9, frac_l0=0.01, frac_mixed=0.05)
+    assert (lg.levels == 4).sum() > 10 and all((lg.levels == l).any() for l in range(5))
     active = np.arange(lg.levels.size)
     for st, dt in ((0, 0.1), (1, 0.02)):
         pkg.sweep_levels(dev, dl, active, st, pkg.PdeParams(dt, 1, False))

@@ -123,13 +123,13 @@ class LevelGrid:
     """A multires grid for the oracle: per level l a SparseGrid with block edge
     16 >> l over one block grid, holding the blocks whose level is l."""
 
-    def __init__(self, pkg, orc, s, n_levels, seed, lo=-1.0, hi=1.0, const=None, frac_l0=0.08):
+    def __init__(self, pkg, orc, s, n_levels, seed, lo=-1.0, hi=1.0, const=None, frac_l0=0.08, frac_mixed=0.5):
         rng = np.random.default_rng(seed)
         nb = [(v + 15) // 16 for v in s]
         n = nb[0] * nb[1] * nb[2]
         m = np.full(n, n_levels - 1, np.uint8)
         u = rng.random(n)
-        m[u < 0.5] = rng.integers(1, n_levels, size=int((u < 0.5).sum()))
+        m[u < frac_mixed] = rng.integers(1, n_levels, size=int((u < frac_mixed).sum()))
         m[u < frac_l0] = 0
         orc.orc_regularize(m, nb[0], nb[1], nb[2], n_levels)
         self.s, self.nb, self.n_levels, self.levels = s, nb, n_levels, m

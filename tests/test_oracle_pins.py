@@ -163,3 +163,36 @@ def test_bilaplacian_sweeps_match_composed_laplacians(pkg, orc):
     # (measured: Jacobi schedule 3.8e-3, dropped diagonal taps 6.5e-2)
     assert np.max(np.abs(got - dense_sweeps(F0, dt, iters, schedule="jacobi"))) > 100 * max(err, 1e-7)
     assert np.max(np.abs(got - dense_sweeps(F0, dt, iters, drop_diagonal=True))) > 100 * max(err, 1e-7)
+
+
+def morton3_py(x, y, z):
+    """Bit interleave x -> bit 0, y -> bit 1, z -> bit 2 (multires/sort.rs:8-22),
+    written independently of the oracle."""
+    k = 0
+    for i in range(21):
+        k |= ((x >> i) & 1) << (3 * i) | ((y >> i) & 1) << (3 * i + 1) | ((z >> i) & 1) << (3 * i + 2)
+    return k
+
+
+@pytest.mark.parametrize("s,frac,seed", [((96, 80, 112), 0.6, 0), ((512, 512, 512), 0.13, 1), ((33, 17, 50), 1.0, 2)])
+def test_color_bucket_properties(pkg, orc, s, frac, seed):
+    """build_color_buckets (solver.rs:247-267) by its properties: bucket c
+    holds exactly the active blocks whose coordinate parity triple is
+    (c & 1, c >> 1 & 1, c >> 2 & 1), buckets are in colour order, each is
+    sorted by strictly increasing morton3 of the block coordinates, and the
+    buckets together are a permutation of the active list."""
+    g = pkg.SparseGrid("c", *s, 0.0, 1.0)
+    rng = np.random.default_rng(seed)
+    active = np.nonzero(rng.random(g.n_blocks) < frac)[0].astype(np.int64)
+    rng.shuffle(active)
+    out, counts = orc.color_buckets(g, active)
+    assert counts.sum() == active.size
+    assert np.array_equal(np.sort(out), np.sort(active))
+    off = 0
+    for c in range(8):
+        part = out[off:off + counts[c]]
+        off += counts[c]
+        bx, by, bz = part % g.nx, (part // g.nx) % g.ny, part // g.nxy
+        assert np.all(((bx & 1) | (by & 1) << 1 | (bz & 1) << 2) == c)
+        keys = [morton3_py(int(x), int(y), int(z)) for x, y, z in zip(bx, by, bz)]
+        assert all(a < b for a, b in zip(keys, keys[1:]))
