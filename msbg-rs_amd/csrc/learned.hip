@@ -15,9 +15,10 @@
 //
 // Every convolution -- forward, and the data gradient (the same kernel with
 // the flipped, channel-transposed weights and offset -off-2) -- is an implicit
-// GEMM: a wave computes 16 output cells x 16 channels with 27 MFMA
-// 16x16x16 steps (one per tap, K = 16 input channels).  The weight gradient
-// is a second GEMM with K running over cells.
+// GEMM on the 16x16x32 BF16 MFMA: 16 output cells x 16 channels per MFMA
+// chain of 14 tap pairs (K = 16 input channels x 2 taps), the input planes
+// staged in LDS.  The weight gradient is a second GEMM with K running over
+// cells.
 #include <cmath>
 #include <vector>
 
@@ -28,6 +29,8 @@ namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 
 constexpr int kLP = 27 * 16 * 16 + 16;  // parameters per layer (padded)
 constexpr int kParams = 4 * kLP;
@@ -41,92 +44,166 @@ __device__ __forceinline__ uint16_t to_bf16(float v) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-struct ConvArgs {
+// ---------------------------------------------------------------------------
+// Convolution as an implicit GEMM on gfx950's 16x16x32 BF16 MFMA.  A
+// work-group (4 waves) computes kZS output z-planes of one tile: it stages the
+// kZS + 2 input planes the 3^3 taps reach into LDS (zero planes past the
+// tile's z range), then each wave takes kCG groups of 16 output cells at a
+// time through the 14 tap pairs (K = 32 = 16 input channels x 2 taps; the
+// 27th tap pairs with zeros) or, for the 2-channel input layer, 2 MFMAs of
+// 16 taps x 2 channels.  Lane layout of v_mfma_f32_16x16x32_bf16: A[m][k] and
+// B[k][n] are held as 8 consecutive k by lane m (n) + 16 (k / 8), D[m][n] as
+// element m % 4 of lane n + 16 (m / 4).  The B operands (weights) are
+// arranged per lane by prep_wb_kernel and kept in registers for the whole
+// work-group.  Each group's result is one MFMA chain in tap order, so its
+// bits do not depend on kZS or kCG.
+// ---------------------------------------------------------------------------
+constexpr int kZS = 4;   // output z-planes per work-group
+constexpr int kCG = 4;   // 16-cell groups per wave pass
+constexpr int kTP = 14;  // tap pairs of a 16-channel layer
+constexpr int kWB = kTP * 64;  // B-operand vectors (16 B) per layer
+
+struct Conv2Args {
     const uint16_t *in;
     const int32_t *idx;  // tile index into `in` per batch position, or null
     int sin, sout, off;
-    const uint16_t *w;   // [27][16][16] bf16
+    const u4v *wb;       // [kTP][64] B operands of this layer
     const float *bias;   // [16] or null
     int relu;
     const uint16_t *mask;  // [tile][sout^3][16]: output zeroed where mask <= 0, or null
     void *out;             // bf16 [tile][sout^3][16], or f32 [tile][sout^3] (channel 0)
-    int groups;            // ceil(sout^3 / 16)
-    int n_tiles;
+    int slabs;             // ceil(sout / kZS)
 };
 
-// MFMA 16x16x16 bf16 lane layout: A[m][k] from lane m + 16 (k / 4), B[k][n]
-// from lane n + 16 (k / 4), D[m][n] in lane n + 16 (m / 4), element m % 4.
-//
-// A wave computes kConvGroups groups of 16 output cells x 16 channels: per
-// tap it loads the tap's weights once and issues one MFMA per group into that
-// group's own accumulator, so the groups' 27-step MFMA chains interleave.
-// Each group sees the same MFMA sequence as alone, so its bits do not depend
-// on kConvGroups.
-constexpr int kConvGroups = 4;
-
 template <int CIN, bool OUTF>
-__global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
-    constexpr int G = kConvGroups;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int gpw = (a.groups + G - 1) / G;  // wave slots per tile
-    const int tile = wave / gpw, grp0 = (wave - tile * gpw) * G;
-    if (tile >= a.n_tiles) return;  // wave-uniform
-    const int ncell = a.sout * a.sout * a.sout;
-    const int m = lane & 15, kg = lane >> 4;
-    int qx[G], qy[G], qz[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        int q = (grp0 + g) * 16 + m;
-        if (q >= ncell) q = ncell - 1;  // past the last cell: computed, not stored
-        qx[g] = q % a.sout, qy[g] = (q / a.sout) % a.sout, qz[g] = q / (a.sout * a.sout);
+__global__ __launch_bounds__(256) void conv2_kernel(Conv2Args a) {
+    extern __shared__ u4v lds[];
+    constexpr int NTP = CIN == 16 ? kTP : 2;
+    const int tile = blockIdx.x / a.slabs, slab = blockIdx.x - tile * a.slabs;
+    const int z0 = slab * kZS, zn = min(kZS, a.sout - z0);
+    const int sin = a.sin, sout = a.sout, off = a.off;
+    // stage input planes z0 + off .. z0 + off + zn + 1 (16-byte vectors)
+    const int plane_v = sin * sin * CIN / 8;
+    const int64_t tin = (int64_t)(a.idx ? a.idx[tile] : tile) * sin * sin * sin;
+    const u4v *src = reinterpret_cast<const u4v *>(a.in + tin * CIN);
+    for (int i = threadIdx.x; i < (zn + 2) * plane_v; i += 256) {
+        const int p = i / plane_v, r = i - p * plane_v;
+        const int iz = z0 + off + p;
+        u4v v = {0u, 0u, 0u, 0u};
+        if (iz >= 0 && iz < sin) v = src[(int64_t)iz * plane_v + r];
+        lds[i] = v;
     }
-    const int64_t in_base = (int64_t)(a.idx ? a.idx[tile] : tile) * a.sin * a.sin * a.sin;
-    f4 acc[G];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u4v wv[NTP];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < 27; ++t) {
-        const uint16_t *wp = a.w + (t * 16 + 4 * kg) * 16 + m;
-        const s4 bv = {(short)wp[0], (short)wp[16], (short)wp[32], (short)wp[48]};
-        const int ox = a.off + t % 3, oy = a.off + (t / 3) % 3, oz = a.off + t / 9;
-        s4 av[G];
+    for (int tp = 0; tp < NTP; ++tp) wv[tp] = a.wb[tp * 64 + lane];
+    __syncthreads();
+    const int m = lane & 15, kb = lane >> 4;
+    const int s2 = sout * sout, ncell = zn * s2, groups = (ncell + 15) / 16;
+    const float bias = a.bias ? a.bias[m] : 0.0f;
+    const uint32_t *L32 = reinterpret_cast<const uint32_t *>(lds);
+    for (int g0 = wave * kCG; g0 < groups; g0 += 4 * kCG) {
+        int cx[kCG], cy[kCG], cz[kCG];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int ix = qx[g] + ox, iy = qy[g] + oy, iz = qz[g] + oz;
-            av[g] = s4{0, 0, 0, 0};
-            if ((unsigned)ix < (unsigned)a.sin && (unsigned)iy < (unsigned)a.sin && (unsigned)iz < (unsigned)a.sin) {
-                const int64_t cell = in_base + ((int64_t)iz * a.sin + iy) * a.sin + ix;
-                if (CIN == 16) {
-                    const uint2 v = *(const uint2 *)(a.in + cell * 16 + 4 * kg);
-                    av[g] = s4{(short)(v.x & 0xffff), (short)(v.x >> 16), (short)(v.y & 0xffff), (short)(v.y >> 16)};
-                } else if (kg == 0) {
-                    const uint32_t v = *(const uint32_t *)(a.in + cell * 2);
-                    av[g] = s4{(short)(v & 0xffff), (short)(v >> 16), 0, 0};
+        for (int g = 0; g < kCG; ++g) {
+            int q = (g0 + g) * 16 + m;
+            if (q >= ncell) q = ncell - 1;  // past the last cell: computed, not stored
+            cz[g] = q / s2;
+            const int r = q - cz[g] * s2;
+            cy[g] = r / sout;
+            cx[g] = r - cy[g] * sout;
+        }
+        f4 acc[kCG];
+#pragma unroll
+        for (int g = 0; g < kCG; ++g) acc[g] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int tp = 0; tp < NTP; ++tp) {
+            u4v av[kCG];
+#pragma unroll
+            for (int g = 0; g < kCG; ++g) {
+                if constexpr (CIN == 16) {
+                    const int t = 2 * tp + (kb >> 1);
+                    u4v v = {0u, 0u, 0u, 0u};
+                    if (t < 27) {
+                        const int ix = cx[g] + off + t % 3, iy = cy[g] + off + (t / 3) % 3, iz = cz[g] + t / 9;
+                        if ((unsigned)ix < (unsigned)sin && (unsigned)iy < (unsigned)sin)
+                            v = lds[((iz * sin + iy) * sin + ix) * 2 + (kb & 1)];
+                    }
+                    av[g] = v;
+                } else {
+                    // taps 16 tp + 4 kb + j, j < 4, both channels (4 bytes each)
+                    u4v v = {0u, 0u, 0u, 0u};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int t = 16 * tp + 4 * kb + j;
+                        if (t < 27) {
+                            const int ix = cx[g] + off + t % 3, iy = cy[g] + off + (t / 3) % 3, iz = cz[g] + t / 9;
+                            if ((unsigned)ix < (unsigned)sin && (unsigned)iy < (unsigned)sin)
+                                v[j] = L32[(iz * sin + iy) * sin + ix];
+                        }
+                    }
+                    av[g] = v;
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kCG; ++g)
+                acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av[g]),
+                                                                 __builtin_bit_cast(bf8, wv[tp]), acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < kCG; ++g) {
+            const int grp = g0 + g;
+            if (grp >= groups) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int cl = grp * 16 + 4 * kb + j;
+                if (cl >= ncell) continue;
+                float v = acc[g][j] + bias;
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                const int64_t o = ((int64_t)tile * sout + z0) * s2 + cl;
+                if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
+                if (OUTF) {
+                    if (m == 0) ((float *)a.out)[o] = v;
+                } else {
+                    ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
                 }
             }
         }
-#pragma unroll
-        for (int g = 0; g < G; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[g], bv, acc[g], 0, 0, 0);
     }
-    const float bias = a.bias ? a.bias[m] : 0.0f;
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const int grp = grp0 + g;
-        if (grp >= a.groups) continue;
-        for (int j = 0; j < 4; ++j) {
-            const int cell = grp * 16 + 4 * kg + j;
-            if (cell >= ncell) continue;
-            float v = acc[g][j] + bias;
-            if (a.relu) v = v > 0.0f ? v : 0.0f;
-            const int64_t o = (int64_t)tile * ncell + cell;
-            if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
-            if (OUTF) {
-                if (m == 0) ((float *)a.out)[o] = v;
-            } else {
-                ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
-            }
+}
+
+// B operands of the four layers for conv2_kernel, forward (wb) and data
+// gradient (wbf: layer l's flipped, channel-transposed weights, layers 1..3):
+// lane (n, kb) of tap pair tp holds 8 bf16 at k = 8 kb .. 8 kb + 7.  16-channel
+// layers: k = 16 (tap - 2 tp) + c, tap = 2 tp + (kb >> 1); the 2-channel layer
+// 0: k = 2 (tap - 16 tp) + c, tap = 16 tp + 4 kb + (i >> 1).  Taps past 26 are 0.
+__global__ void prep_wb_kernel(const float *__restrict__ params, u4v *wb, u4v *wbf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * kWB) return;
+    const int l = i / kWB, r = i % kWB, tp = r / 64, lane = r % 64;
+    const int n = lane & 15, kb = lane >> 4;
+    const float *W = params + (int64_t)l * kLP;  // W[t][c][o]
+    uint16_t f[8], b[8];
+    for (int e = 0; e < 8; ++e) {
+        int t, c;
+        if (l == 0) {
+            t = 16 * tp + 4 * kb + (e >> 1);
+            c = e & 1;
+            if (tp >= 2) t = 27;
+        } else {
+            t = 2 * tp + (kb >> 1);
+            c = (kb & 1) * 8 + e;
         }
+        f[e] = t < 27 ? to_bf16(W[(t * 16 + c) * 16 + n]) : (uint16_t)0;
+        b[e] = (t < 27 && l > 0) ? to_bf16(W[((26 - t) * 16 + n) * 16 + c]) : (uint16_t)0;
     }
+    u4v vf, vb;
+    for (int e = 0; e < 4; ++e) {
+        vf[e] = (uint32_t)f[2 * e] | (uint32_t)f[2 * e + 1] << 16;
+        vb[e] = (uint32_t)b[2 * e] | (uint32_t)b[2 * e + 1] << 16;
+    }
+    wb[i] = vf;
+    wbf[i] = vb;
 }
 
 struct WgArgs {
@@ -141,8 +218,10 @@ struct WgArgs {
 };
 
 // Weight / bias gradient partials of one layer: wave (tile, chunk) sums over
-// its cells dW[t][c][o] += X[p + off + d_t][c] G[p][o] (MFMA with M = c,
-// N = o, K = 16 cells) and db[o] += G[p][o].
+// its cells dW[t][c][o] += X[p + off + d_t][c] G[p][o] (MFMA 16x16x16 with
+// M = c, N = o, K = 16 cells: A[m][k] from lane m + 16 (k / 4), B[k][n] from
+// lane n + 16 (k / 4), D[m][n] in element m % 4 of lane n + 16 (m / 4)) and
+// db[o] += G[p][o].
 template <int CIN>
 __global__ __launch_bounds__(256) void wgrad_kernel(WgArgs a) {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -201,18 +280,6 @@ __global__ void reduce_partials_kernel(const float *__restrict__ part, int n, fl
     float s = 0.0f;
     for (int w = 0; w < n; ++w) s += part[(int64_t)w * kLP + k];
     grad[k] = s;
-}
-
-// bf16 weights and their flipped, channel-transposed copy for the data gradient:
-// wf[l][26 - t][o][c] = w[l][t][c][o].
-__global__ void prep_weights_kernel(const float *__restrict__ params, uint16_t *w, uint16_t *wf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 4 * kTap) return;
-    const int l = i / kTap, r = i % kTap;
-    const int t = r / 256, c = (r / 16) % 16, o = r % 16;
-    const uint16_t v = to_bf16(params[(int64_t)l * kLP + r]);
-    w[i] = v;
-    wf[l * kTap + ((26 - t) * 16 + o) * 16 + c] = v;
 }
 
 // Loss (y - target)^2 / (2 n 4096): G4 = (y - target) / (n 4096) in channel 0,
@@ -417,7 +484,7 @@ using namespace msbg;
 
 struct msbg_learned {
     float *d_params = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr;
-    uint16_t *d_w = nullptr, *d_wf = nullptr;
+    msbg::u4v *d_wb = nullptr, *d_wbf = nullptr;  // MFMA B operands, forward and data gradient
     float out_scale = 1.0f;
     int adam_t = 0;
     // workspace for `cap` tiles
@@ -471,32 +538,31 @@ int ensure_ws(msbg_learned *n, int64_t tiles, bool train) {
 }
 
 template <int CIN, bool OUTF>
-void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int sout, int off, const uint16_t *w,
+void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int sout, int off, const u4v *wb,
           const float *bias, int relu, const uint16_t *mask, void *out, int n) {
-    ConvArgs a;
+    Conv2Args a;
     a.in = in;
     a.idx = idx;
     a.sin = sin;
     a.sout = sout;
     a.off = off;
-    a.w = w;
+    a.wb = wb;
     a.bias = bias;
     a.relu = relu;
     a.mask = mask;
     a.out = out;
-    a.groups = (sout * sout * sout + 15) / 16;
-    a.n_tiles = n;
-    const int64_t waves = (int64_t)((a.groups + kConvGroups - 1) / kConvGroups) * n;
-    conv_kernel<CIN, OUTF><<<(int)((waves + 3) / 4), 256, 0, s>>>(a);
+    a.slabs = (sout + kZS - 1) / kZS;
+    const size_t lds = (size_t)(kZS + 2) * sin * sin * CIN * 2;
+    conv2_kernel<CIN, OUTF><<<n * a.slabs, 256, lds, s>>>(a);
 }
 
 void forward(msbg_learned *net, hipStream_t s, const uint16_t *x0, const int32_t *idx, int n) {
-    const uint16_t *W = net->d_w;
+    const u4v *W = net->d_wb;
     const float *P = net->d_params;
     conv<2, false>(s, x0, idx, 20, 18, 0, W, P + 0 * kLP + kTap, 1, nullptr, net->A1, n);
-    conv<16, false>(s, net->A1, nullptr, 18, 16, 0, W + kTap, P + 1 * kLP + kTap, 1, nullptr, net->A2, n);
-    conv<16, false>(s, net->A2, nullptr, 16, 16, -1, W + 2 * kTap, P + 2 * kLP + kTap, 1, nullptr, net->A3, n);
-    conv<16, true>(s, net->A3, nullptr, 16, 16, -1, W + 3 * kTap, P + 3 * kLP + kTap, 0, nullptr, net->Y, n);
+    conv<16, false>(s, net->A1, nullptr, 18, 16, 0, W + kWB, P + 1 * kLP + kTap, 1, nullptr, net->A2, n);
+    conv<16, false>(s, net->A2, nullptr, 16, 16, -1, W + 2 * kWB, P + 2 * kLP + kTap, 1, nullptr, net->A3, n);
+    conv<16, true>(s, net->A3, nullptr, 16, 16, -1, W + 3 * kWB, P + 3 * kLP + kTap, 0, nullptr, net->Y, n);
 }
 
 template <int CIN>
@@ -526,19 +592,19 @@ void loss_grad(msbg_learned *net, hipStream_t s, const uint16_t *x0, const float
     const int nb = n * 16;
     loss_kernel<<<nb, 256, 0, s>>>(net->Y, target, idx, n, (float)inv, net->G4, net->loss_part);
     loss_reduce_kernel<<<1, 64, 0, s>>>(net->loss_part, nb, 0.5 * inv, net->d_loss, loss_f);
-    const uint16_t *Wf = net->d_wf;
+    const u4v *Wf = net->d_wbf;
     float *g = net->d_grad;
     wgrad<16>(net, s, net->A3, nullptr, 16, 16, -1, net->G4, n, g + 3 * kLP);
-    conv<16, false>(s, net->G4, nullptr, 16, 16, -1, Wf + 3 * kTap, nullptr, 0, net->A3, net->G3, n);
+    conv<16, false>(s, net->G4, nullptr, 16, 16, -1, Wf + 3 * kWB, nullptr, 0, net->A3, net->G3, n);
     wgrad<16>(net, s, net->A2, nullptr, 16, 16, -1, net->G3, n, g + 2 * kLP);
-    conv<16, false>(s, net->G3, nullptr, 16, 16, -1, Wf + 2 * kTap, nullptr, 0, net->A2, net->G2, n);
+    conv<16, false>(s, net->G3, nullptr, 16, 16, -1, Wf + 2 * kWB, nullptr, 0, net->A2, net->G2, n);
     wgrad<16>(net, s, net->A1, nullptr, 18, 16, 0, net->G2, n, g + 1 * kLP);
-    conv<16, false>(s, net->G2, nullptr, 16, 18, -2, Wf + 1 * kTap, nullptr, 0, net->A1, net->G1, n);
+    conv<16, false>(s, net->G2, nullptr, 16, 18, -2, Wf + 1 * kWB, nullptr, 0, net->A1, net->G1, n);
     wgrad<2>(net, s, x0, idx, 20, 18, 0, net->G1, n, g + 0 * kLP);
 }
 
 void prep(msbg_learned *net, hipStream_t s) {
-    prep_weights_kernel<<<blocks_for(4 * kTap), 256, 0, s>>>(net->d_params, net->d_w, net->d_wf);
+    prep_wb_kernel<<<blocks_for(4 * kWB), 256, 0, s>>>(net->d_params, net->d_wb, net->d_wbf);
 }
 
 }  // namespace
@@ -618,8 +684,8 @@ int msbg_learned_new(msbg_learned **out) {
     if (e == hipSuccess) e = hipMalloc(&n->d_m, kParams * 4);
     if (e == hipSuccess) e = hipMalloc(&n->d_v, kParams * 4);
     if (e == hipSuccess) e = hipMalloc(&n->d_grad, kParams * 4);
-    if (e == hipSuccess) e = hipMalloc(&n->d_w, 4 * kTap * 2);
-    if (e == hipSuccess) e = hipMalloc(&n->d_wf, 4 * kTap * 2);
+    if (e == hipSuccess) e = hipMalloc(&n->d_wb, 4 * kWB * sizeof(u4v));
+    if (e == hipSuccess) e = hipMalloc(&n->d_wbf, 4 * kWB * sizeof(u4v));
     if (e == hipSuccess) e = hipMalloc(&n->d_loss, sizeof(double));
     if (e == hipSuccess) e = hipMemset(n->d_params, 0, kParams * 4);
     if (e == hipSuccess) e = hipMemset(n->d_m, 0, kParams * 4);
@@ -640,8 +706,8 @@ int msbg_learned_free(msbg_learned *n) {
     (void)hipFree(n->d_m);
     (void)hipFree(n->d_v);
     (void)hipFree(n->d_grad);
-    (void)hipFree(n->d_w);
-    (void)hipFree(n->d_wf);
+    (void)hipFree(n->d_wb);
+    (void)hipFree(n->d_wbf);
     (void)hipFree(n->d_loss);
     delete n;
     return MSBG_OK;

@@ -41,7 +41,7 @@ class FlipStep:
 
     def run(self, pos, vel, mass, r_p=2.0, alpha=0.95, storage=Storage.F32, refine=False, omega=1.9,
             sweeps_per_check=4, max_sweeps=20000, tol=1e-6, learned=None, phi=None, multigrid=False,
-            compensated=False):
+            compensated=False, direct=False):
         """One step.  pos / vel / mass: device arrays (n x 3, n x 3, n); the
         solve's arguments are PoissonBand.solve's.  compensated (refine or
         multigrid): the pressure is the compensated pair p + p_lo of two FP32
@@ -78,7 +78,8 @@ class FlipStep:
         p = band.field(0)
         p_lo = band.field(0) if compensated else None
         st = band.solve(p, b, storage=storage, refine=refine, omega=omega, sweeps_per_check=sweeps_per_check,
-                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi, multigrid=multigrid, x_lo=p_lo)
+                        max_sweeps=max_sweeps, tol=tol, learned=learned, phi=phi, multigrid=multigrid, x_lo=p_lo,
+                        direct=direct)
         rec(4)
         self.mac.subtract_gradient(layout, p, active, -1.0, p_lo=p_lo)
         rec(5)
