@@ -272,6 +272,20 @@ int msbg_bucket_indices(const int32_t *d_bids, int64_t n, int64_t n_blocks, int3
  * up to nx*ny*nz entries).  P2G uses r_scan = r_p + 1 (p2g.rs:104-107). */
 int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, float r_scan,
                        int32_t *d_out, int64_t *n_out);
+/* No counterpart in the reference.  msbg_active_blocks for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): same arguments, same results, and
+ * no hipMalloc / hipFree / allocating library call.  Its scratch (block
+ * flags, ranks and scan partials) is one fixed device array of the library
+ * per GPU (2.1 MB, held until the process ends), so the block grid is limited
+ * to MSBG_NOALLOC_MAX_BLOCKS blocks (a 1024^3 domain of 16^3 blocks); larger
+ * grids return MSBG_EINVAL before any device work.  Every call on a GPU shares
+ * that array: calls that overlap on one GPU (several host threads, or streams
+ * set in between) race.  Synchronizes the library stream (the count is read
+ * back). */
+#define MSBG_NOALLOC_MAX_BLOCKS 262144
+int msbg_active_blocks_noalloc(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, float r_scan,
+                               int32_t *d_out, int64_t *n_out);
 /* d_dst[i] = d_src[d_perm[i]] for rows of row_bytes (a multiple of 4): the
  * payload gather of bucket_particles (fluid/mod.rs:84-89). */
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst);

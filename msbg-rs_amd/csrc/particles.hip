@@ -1,6 +1,7 @@
 // Particle-side kernels: Morton keys, the Morton sort, block bucketing
 // (bucket_indices) and active-block compaction (active_blocks).
 #include <algorithm>
+#include <mutex>
 
 #include "primitives.hpp"
 
@@ -137,6 +138,36 @@ int active_blocks_into(const float *d_pos, int64_t n, float r_scan, int32_t nx, 
     return MSBG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of msbg_active_blocks_noalloc: the flags and ranks of up to
+// kNoAllocMaxBlocks blocks and the scan's tile sums, one copy per GPU for the
+// life of the process (2.1 MB).  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxBlocks = MSBG_NOALLOC_MAX_BLOCKS;
+constexpr int64_t kNoAllocSumWords = 1024;  // >= scan_sums_words(kNoAllocMaxBlocks) = 129
+constexpr int64_t kNoAllocWords = 2 * kNoAllocMaxBlocks + kNoAllocSumWords;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t *g_scratch[kMaxDevices] = {nullptr};
+
+// The current device's copy of `scratch`, resolved on its first use there.
+int noalloc_scratch(uint32_t **out) {
+    int dev = 0;
+    MSBG_HIP(hipGetDevice(&dev));
+    MSBG_CHECK(dev >= 0 && dev < kMaxDevices, MSBG_EINVAL, "no-alloc scratch: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        MSBG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (uint32_t *)ptr;
+    }
+    *out = g_scratch[dev];
+    return MSBG_OK;
+}
+
 int bits_for(int64_t v) {
     int b = 1;
     while (b < 32 && ((int64_t)1 << b) <= v) ++b;
@@ -227,6 +258,25 @@ int msbg_active_blocks(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, i
     uint32_t *sums = (uint32_t *)workspace(0, sizeof(uint32_t) * scan_sums_words(nb), &rc);
     if (rc) return rc;
     return active_blocks_into(d_pos, n, r_scan, nx, ny, nz, flags, flags + nb, sums, d_out, n_out);
+}
+
+int msbg_active_blocks_noalloc(int64_t sx, int64_t sy, int64_t sz, const float *d_pos, int64_t n, float r_scan,
+                               int32_t *d_out, int64_t *n_out) {
+    MSBG_CHECK(n_out && d_out, MSBG_EINVAL, "active_blocks_noalloc: null output");
+    MSBG_CHECK(sx > 0 && sy > 0 && sz > 0 && n >= 0 && (n == 0 || d_pos), MSBG_EINVAL,
+               "active_blocks_noalloc: bad argument");
+    const int64_t nx = (sx + 15) >> 4, ny = (sy + 15) >> 4, nz = (sz + 15) >> 4;
+    MSBG_CHECK(nx * ny * nz <= kNoAllocMaxBlocks, MSBG_EINVAL,
+               "active_blocks_noalloc: the block grid exceeds MSBG_NOALLOC_MAX_BLOCKS (262,144 blocks, a 1024^3 "
+               "domain of 16^3 blocks); use msbg_active_blocks");
+    static_assert(kNoAllocSumWords >= (kNoAllocMaxBlocks + 2047) / 2048 + 1, "scan tile sums do not fit");
+    uint32_t *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    const int64_t nb = nx * ny * nz;
+    MSBG_CHECK(scan_sums_words(nb) <= kNoAllocSumWords, MSBG_EINVAL, "active_blocks_noalloc: scan sums overflow");
+    return active_blocks_into(d_pos, n, r_scan, (int32_t)nx, (int32_t)ny, (int32_t)nz, base, base + kNoAllocMaxBlocks,
+                              base + 2 * kNoAllocMaxBlocks, d_out, n_out);
 }
 
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst) {
