@@ -53,13 +53,21 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 // 27th tap pairs with zeros) or, for the 2-channel input layer, 2 MFMAs of
 // 16 taps x 2 channels.  Lane layout of v_mfma_f32_16x16x32_bf16: A[m][k] and
 // B[k][n] are held as 8 consecutive k by lane m (n) + 16 (k / 8), D[m][n] as
-// element m % 4 of lane n + 16 (m / 4).  The B operands (weights) are
-// arranged per lane by prep_wb_kernel and kept in registers for the whole
-// work-group.  Each group's result is one MFMA chain in tap order, so its
-// bits do not depend on kZS or kCG.
+// element m % 4 of lane n + 16 (m / 4).  A is the weights (M = output
+// channels), B the input cells (N = 16 cells), so lane c + 16 j ends with
+// channels 4 j .. 4 j + 3 of cell c and stores them as one 8-byte vector (a
+// wave writes 512 contiguous bytes).  The weight operands are arranged per
+// lane by prep_wb_kernel and kept in registers for the whole work-group.
+// Each group's result is one MFMA chain in tap order, so its bits do not
+// depend on kZS, kCG or the work-group size.
 // ---------------------------------------------------------------------------
+// Tiles per inference pass: 512 tiles' activations (163 MB) stay in the
+// 256 MB Infinity Cache from one layer to the next; 4096-tile passes stream
+// them through HBM and measured 30% slower per tile.
+constexpr int64_t kForwardChunk = 512;
 constexpr int kZS = 4;   // output z-planes per work-group
 constexpr int kCG = 4;   // 16-cell groups per wave pass
+constexpr int kConvThreads = 512;  // 8 waves share one work-group's staged planes
 constexpr int kTP = 14;  // tap pairs of a 16-channel layer
 constexpr int kWB = kTP * 64;  // B-operand vectors (16 B) per layer
 
@@ -76,7 +84,7 @@ struct Conv2Args {
 };
 
 template <int CIN, bool OUTF>
-__global__ __launch_bounds__(256) void conv2_kernel(Conv2Args a) {
+__global__ __launch_bounds__(kConvThreads) void conv2_kernel(Conv2Args a) {
     extern __shared__ u4v lds[];
     constexpr int NTP = CIN == 16 ? kTP : 2;
     const int tile = blockIdx.x / a.slabs, slab = blockIdx.x - tile * a.slabs;
@@ -86,7 +94,7 @@ __global__ __launch_bounds__(256) void conv2_kernel(Conv2Args a) {
     const int plane_v = sin * sin * CIN / 8;
     const int64_t tin = (int64_t)(a.idx ? a.idx[tile] : tile) * sin * sin * sin;
     const u4v *src = reinterpret_cast<const u4v *>(a.in + tin * CIN);
-    for (int i = threadIdx.x; i < (zn + 2) * plane_v; i += 256) {
+    for (int i = threadIdx.x; i < (zn + 2) * plane_v; i += kConvThreads) {
         const int p = i / plane_v, r = i - p * plane_v;
         const int iz = z0 + off + p;
         u4v v = {0u, 0u, 0u, 0u};
@@ -100,9 +108,12 @@ __global__ __launch_bounds__(256) void conv2_kernel(Conv2Args a) {
     __syncthreads();
     const int m = lane & 15, kb = lane >> 4;
     const int s2 = sout * sout, ncell = zn * s2, groups = (ncell + 15) / 16;
-    const float bias = a.bias ? a.bias[m] : 0.0f;
+    float bias[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bias[j] = a.bias ? a.bias[4 * kb + j] : 0.0f;
     const uint32_t *L32 = reinterpret_cast<const uint32_t *>(lds);
-    for (int g0 = wave * kCG; g0 < groups; g0 += 4 * kCG) {
+    constexpr int kWaves = kConvThreads / 64;
+    for (int g0 = wave * kCG; g0 < groups; g0 += kWaves * kCG) {
         int cx[kCG], cy[kCG], cz[kCG];
 #pragma unroll
         for (int g = 0; g < kCG; ++g) {
@@ -147,26 +158,35 @@ __global__ __launch_bounds__(256) void conv2_kernel(Conv2Args a) {
             }
 #pragma unroll
             for (int g = 0; g < kCG; ++g)
-                acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, av[g]),
-                                                                 __builtin_bit_cast(bf8, wv[tp]), acc[g], 0, 0, 0);
+                acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, wv[tp]),
+                                                                 __builtin_bit_cast(bf8, av[g]), acc[g], 0, 0, 0);
         }
 #pragma unroll
         for (int g = 0; g < kCG; ++g) {
             const int grp = g0 + g;
-            if (grp >= groups) continue;
+            const int cl = grp * 16 + m;  // this lane's cell; channels 4 kb .. 4 kb + 3
+            if (grp >= groups || cl >= ncell) continue;
+            const int64_t o = ((int64_t)tile * sout + z0) * s2 + cl;
+            float v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int cl = grp * 16 + 4 * kb + j;
-                if (cl >= ncell) continue;
-                float v = acc[g][j] + bias;
-                if (a.relu) v = v > 0.0f ? v : 0.0f;
-                const int64_t o = ((int64_t)tile * sout + z0) * s2 + cl;
-                if (a.mask && !(from_bf16(a.mask[o * 16 + m]) > 0.0f)) v = 0.0f;
-                if (OUTF) {
-                    if (m == 0) ((float *)a.out)[o] = v;
-                } else {
-                    ((uint16_t *)a.out)[o * 16 + m] = to_bf16(v);
-                }
+                v[j] = acc[g][j] + bias[j];
+                if (a.relu) v[j] = v[j] > 0.0f ? v[j] : 0.0f;
+            }
+            if (a.mask) {
+                const uint2 mk = *reinterpret_cast<const uint2 *>(a.mask + o * 16 + 4 * kb);
+                const uint16_t mh[4] = {(uint16_t)(mk.x & 0xffffu), (uint16_t)(mk.x >> 16), (uint16_t)(mk.y & 0xffffu),
+                                        (uint16_t)(mk.y >> 16)};
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (!(from_bf16(mh[j]) > 0.0f)) v[j] = 0.0f;
+            }
+            if (OUTF) {
+                if (kb == 0) ((float *)a.out)[o] = v[0];
+            } else {
+                const uint2 w = make_uint2((uint32_t)to_bf16(v[0]) | (uint32_t)to_bf16(v[1]) << 16,
+                                           (uint32_t)to_bf16(v[2]) | (uint32_t)to_bf16(v[3]) << 16);
+                *reinterpret_cast<uint2 *>((uint16_t *)a.out + o * 16 + 4 * kb) = w;
             }
         }
     }
@@ -553,7 +573,7 @@ void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int so
     a.out = out;
     a.slabs = (sout + kZS - 1) / kZS;
     const size_t lds = (size_t)(kZS + 2) * sin * sin * CIN * 2;
-    conv2_kernel<CIN, OUTF><<<n * a.slabs, 256, lds, s>>>(a);
+    conv2_kernel<CIN, OUTF><<<n * a.slabs, kConvThreads, lds, s>>>(a);
 }
 
 void forward(msbg_learned *net, hipStream_t s, const uint16_t *x0, const int32_t *idx, int n) {
@@ -632,7 +652,7 @@ int correct(msbg_poisson *p, msbg_learned *net, const msbg_grid *phi, float *x, 
         p->d_xtry = xtry;
         p->d_dots = dots;
     }
-    const int64_t chunk = std::min<int64_t>(p->n_active, 512);
+    const int64_t chunk = std::min<int64_t>(p->n_active, kForwardChunk);
     int rc = ensure_ws(net, chunk, false);
     if (rc) return rc;
     uint16_t *x0 = net->X0;
@@ -737,7 +757,7 @@ int msbg_learned_get_params(const msbg_learned *n, float *host_params, float *ou
 
 int msbg_learned_forward(msbg_learned *n, const uint16_t *d_x0, int64_t count, float *d_y) {
     MSBG_CHECK(n && d_x0 && d_y && count >= 0, MSBG_EINVAL, "bad argument");
-    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(count, 1), 512);
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(count, 1), kForwardChunk);
     int rc = ensure_ws(n, chunk, false);
     if (rc) return rc;
     for (int64_t k0 = 0; k0 < count; k0 += chunk) {

@@ -168,6 +168,13 @@ int noalloc_scratch(uint32_t **out) {
     return MSBG_OK;
 }
 
+int morton_sort_async(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm) {
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, d_keys, d_perm);
+    MSBG_HIP(hipGetLastError());
+    return radix_sort_pairs(d_keys, d_perm, n, 30);
+}
+
 int bits_for(int64_t v) {
     int b = 1;
     while (b < 32 && ((int64_t)1 << b) <= v) ++b;
@@ -201,9 +208,7 @@ int msbg_morton_sort(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *
     MSBG_CHECK(n >= 0 && n < ((int64_t)1 << 31), MSBG_EINVAL, "morton sort: n must be below 2^31");
     MSBG_CHECK(n == 0 || (d_pos && d_keys && d_perm), MSBG_EINVAL, "morton sort: null argument");
     if (n == 0) return MSBG_OK;
-    hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, d_keys, d_perm);
-    MSBG_HIP(hipGetLastError());
-    int rc = radix_sort_pairs(d_keys, d_perm, n, 30);
+    int rc = morton_sort_async(d_pos, n, d_keys, d_perm);
     if (rc) return rc;
     MSBG_HIP(hipStreamSynchronize(stream()));
     return MSBG_OK;

@@ -22,6 +22,9 @@ int64_t scan_sums_words(int64_t n);
 // Stable sort of (keys, vals) by the low `key_bits` bits of keys, in place.
 int radix_sort_pairs(uint32_t *keys, uint32_t *vals, int64_t n, int key_bits);
 
+// msbg_morton_sort without the final synchronization (stream-ordered).
+int morton_sort_async(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm);
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {  // multires/sort.rs:8-16, 10 bits
     v &= 0x3ffu;
     v = (v | (v << 16)) & 0x030000ffu;

@@ -162,6 +162,12 @@ __device__ __forceinline__ void fill_tile_dense(float *tile, const GridView &g, 
     }
 }
 
+// Distance of tile coordinate h outside the block's [H, H + 16) (0 inside).
+template <int HALO>
+__device__ __forceinline__ int out_dist(int h) {
+    return h < HALO ? HALO - h : (h >= HALO + 16 ? h - (HALO + 15) : 0);
+}
+
 template <int HALO>
 __device__ __forceinline__ int tile_at(int hx, int hy, int hz) {
     constexpr int HSX = 16 + 2 * HALO;
@@ -191,12 +197,20 @@ __device__ void fill_tile_pitched_generic(float *tile, const TS *fs, const GridV
 // Row-wise pitched fill for blocks whose rows lie inside the domain in x.  A
 // tile row (hy, hz) is resolved once: its 16 interior cells are one block row
 // (four float4 loads, or two float2 loads duplicated for a coarse
-// neighbour), its 2H x-halo cells are loaded one by one.  `skip_edges` leaves
-// out the rows and x-halo cells a 7-point stencil never reads.
-template <int HALO, bool MULTI, int NTHR, typename TS>
+// neighbour), its 2H x-halo cells are loaded one by one.  SKIP leaves out
+// the tile cells the stencil never reads: kSkipCross (7-point: the corner
+// rows and the x-halo of every row outside the block in y or z) or
+// kSkipBiLap (25-tap: a row outside the block in y or z at distance 2 takes
+// no x-halo, at distance 1 only the inner x-halo cell; of the corner rows only
+// the four at distance 1 in both y and z, without x-halo).  An x-halo cell
+// of a row outside the block lives in a diagonal neighbour block, so each
+// skipped one saves a line fetched for a single value.
+constexpr int kSkipNone = 0, kSkipCross = 1, kSkipBiLap = 2;
+
+template <int HALO, bool MULTI, int NTHR, typename TS, int SKIP>
 __device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const GridView &g, const GridView &c,
                                                const int *s_nb, const int *s_nbc, int bx0, int by0, int bz0,
-                                               float empty_f32, bool skip_edges) {
+                                               float empty_f32) {
     constexpr int B = 16, L = 4, HSX = B + 2 * HALO, n = 2 * HALO + 1;
     constexpr int NR = HSX * HSX, NQ = 4 * NR, NH = 2 * HALO * NR;
     constexpr int QI = (NQ + NTHR - 1) / NTHR, HI = (NH + NTHR - 1) / NTHR;
@@ -212,7 +226,8 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const 
         const int r = it >> 2, q = it & 3;
         const int hy = r % HSX, hz = r / HSX;
         const bool yh = hy < HALO || hy >= HALO + B, zh = hz < HALO || hz >= HALO + B;
-        if (skip_edges && yh && zh) continue;
+        if (SKIP == kSkipCross && yh && zh) continue;
+        if (SKIP == kSkipBiLap && yh && zh && (out_dist<HALO>(hy) != 1 || out_dist<HALO>(hz) != 1)) continue;
         qwrite |= 1u << k;
         int ry, rz;
         if (!resolve_axis(y0 + hy - HALO, g.sy - 1, MSBG_BC_NEUMANN, ry) ||
@@ -237,7 +252,12 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const 
         const int r = it / (2 * HALO), j = it % (2 * HALO);
         const int hy = r % HSX, hz = r / HSX;
         const bool yh = hy < HALO || hy >= HALO + B, zh = hz < HALO || hz >= HALO + B;
-        if (skip_edges && (yh || zh)) continue;
+        if (SKIP == kSkipCross && (yh || zh)) continue;
+        if (SKIP == kSkipBiLap && (yh || zh)) {
+            // only rows at distance 1 outside (not corners) take the inner x-halo cell
+            const bool inner = j == HALO - 1 || j == HALO;
+            if ((yh && zh) || !inner || out_dist<HALO>(yh ? hy : hz) != 1) continue;
+        }
         hwrite |= 1u << k;
         const int gx = j < HALO ? x0 - HALO + j : x0 + B + (j - HALO);
         int rx, ry, rz;
@@ -315,8 +335,10 @@ __device__ __forceinline__ void sweep_block16(GridView g, GridView c, TS *__rest
     const int bx0 = (int)(bid % g.nx), by0 = (int)((bid / g.nx) % g.ny), bz0 = (int)(bid / ((int64_t)g.nx * g.ny));
     __syncthreads();
     if (bx0 * 16 + 16 <= g.sx)
-        fill_tile_rows<HALO, MULTI, kSweepThreads, TS>(tile, fslab, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32,
-                                                   STENCIL == MSBG_LAPLACIAN);
+        fill_tile_rows<HALO, MULTI, kSweepThreads, TS,
+                       STENCIL == MSBG_LAPLACIAN ? kSkipCross
+                                                 : (STENCIL == MSBG_BILAPLACIAN ? kSkipBiLap : kSkipNone)>(
+            tile, fslab, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32);
     else
         fill_tile_pitched_generic<HALO, MULTI, TS>(tile, fslab, g, c, s_nb, s_nbc, bx0, by0, bz0, empty_f32);
     __syncthreads();

@@ -1061,7 +1061,7 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
     uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n, &rc);
     if (rc) return rc;
     uint32_t *perm = keys + n;
-    rc = msbg_morton_sort(d_pos, n, keys, perm);
+    rc = morton_sort_async(d_pos, n, keys, perm);
     if (rc) return rc;
     float4 *pay = (float4 *)workspace(7, sizeof(float4) * 2 * n, &rc);
     if (rc) return rc;
@@ -1101,7 +1101,7 @@ int msbg_g2p_bspline(int order, msbg_grid *const *vel, const float *d_pos, int64
     uint32_t *keys = (uint32_t *)workspace(9, sizeof(uint32_t) * 2 * n, &rc);
     if (rc) return rc;
     uint32_t *visit = keys + n;
-    rc = msbg_morton_sort(d_pos, n, keys, visit);
+    rc = morton_sort_async(d_pos, n, keys, visit);
     if (rc) return rc;
     // one lane per particle (n < 2^31, so the chunk count fits the grid)
     const dim3 chunks((unsigned)((n + kG2pThreads - 1) / kG2pThreads)), lanes(kG2pThreads);
