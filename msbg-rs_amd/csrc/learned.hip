@@ -46,20 +46,21 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 
 // ---------------------------------------------------------------------------
 // Convolution as an implicit GEMM on gfx950's 16x16x32 BF16 MFMA.  A
-// work-group (4 waves) computes kZS output z-planes of one tile: it stages the
-// kZS + 2 input planes the 3^3 taps reach into LDS (zero planes past the
-// tile's z range), then each wave takes kCG groups of 16 output cells at a
-// time through the 14 tap pairs (K = 32 = 16 input channels x 2 taps; the
-// 27th tap pairs with zeros) or, for the 2-channel input layer, 2 MFMAs of
-// 16 taps x 2 channels.  Lane layout of v_mfma_f32_16x16x32_bf16: A[m][k] and
-// B[k][n] are held as 8 consecutive k by lane m (n) + 16 (k / 8), D[m][n] as
-// element m % 4 of lane n + 16 (m / 4).  A is the weights (M = output
-// channels), B the input cells (N = 16 cells), so lane c + 16 j ends with
-// channels 4 j .. 4 j + 3 of cell c and stores them as one 8-byte vector (a
-// wave writes 512 contiguous bytes).  The weight operands are arranged per
-// lane by prep_wb_kernel and kept in registers for the whole work-group.
-// Each group's result is one MFMA chain in tap order, so its bits do not
-// depend on kZS, kCG or the work-group size.
+// work-group (8 waves) computes kZS output z-planes of one tile: it stages the
+// kZS + 2 input planes the 3^3 taps reach into LDS as SP x SP planes
+// (SP = sout + 2, zero where the tap falls outside the input), so every tap
+// read is in bounds and no lane tests a coordinate; then each wave takes kCG
+// groups of 16 output cells at a time through the 14 tap pairs (K = 32 = 16
+// input channels x 2 taps; the 27th tap pairs with zeros) or, for the
+// 2-channel input layer, 2 MFMAs of 16 taps x 2 channels.  Lane layout of
+// v_mfma_f32_16x16x32_bf16: A[m][k] and B[k][n] are held as 8 consecutive k by
+// lane m (n) + 16 (k / 8), D[m][n] as element m % 4 of lane n + 16 (m / 4).  A
+// is the weights (M = output channels), B the input cells (N = 16 cells), so
+// lane c + 16 j ends with channels 4 j .. 4 j + 3 of cell c and stores them as
+// one 8-byte vector (a wave writes 512 contiguous bytes).  The weight operands
+// are arranged per lane by prep_wb_kernel and kept in registers for the whole
+// work-group.  Each group's result is one MFMA chain in tap-pair order, so its
+// bits do not depend on kZS, kCG or the work-group size.
 // ---------------------------------------------------------------------------
 // Tiles per inference pass: 512 tiles' activations (163 MB) stay in the
 // 256 MB Infinity Cache from one layer to the next; 4096-tile passes stream
@@ -69,13 +70,23 @@ constexpr int kZS = 4;   // output z-planes per work-group
 constexpr int kCG = 4;   // 16-cell groups per wave pass
 constexpr int kConvThreads = 512;  // 8 waves share one work-group's staged planes
 constexpr int kTP = 14;  // tap pairs of a 16-channel layer
-constexpr int kWB = kTP * 64;  // B-operand vectors (16 B) per layer
+constexpr int kWB = kTP * 64;  // A-operand vectors (16 B) per layer
+
+// Tap of pair tp in lane half h (h = 0: k = 0..15, h = 1: k = 16..31) of a
+// 16-channel layer; taps are t = dx + 3 dy + 9 dz, 27 is the zero tap.
+// Pairs 0..8 are (dx, dy, 0) / (dx, dy, 1), pairs 9..11 (dx, 0, 2) / (dx, 1, 2),
+// pair 12 (0, 2, 2) / (1, 2, 2), pair 13 (2, 2, 2) / zero: within each run of
+// pairs the h = 1 tap is the h = 0 tap moved by one plane, one row or one
+// cell, so a lane's address is a per-lane base plus a constant per pair.
+__host__ __device__ constexpr int tap_of(int tp, int h) {
+    return tp < 9 ? tp + 9 * h : tp < 12 ? 18 + (tp - 9) + 3 * h : tp == 12 ? 24 + h : (h ? 27 : 26);
+}
 
 struct Conv2Args {
     const uint16_t *in;
     const int32_t *idx;  // tile index into `in` per batch position, or null
-    int sin, sout, off;
-    const u4v *wb;       // [kTP][64] B operands of this layer
+    int sin, off;
+    const u4v *wb;       // [kTP][64] A operands of this layer
     const float *bias;   // [16] or null
     int relu;
     const uint16_t *mask;  // [tile][sout^3][16]: output zeroed where mask <= 0, or null
@@ -83,46 +94,71 @@ struct Conv2Args {
     int slabs;             // ceil(sout / kZS)
 };
 
-template <int CIN, bool OUTF>
+// Output edge sout = SP - 2; input cell (off + x, off + y, off + z) of the
+// tile's sin^3 input sits at staged (x, y, z - z0).
+template <int CIN, int SP, bool OUTF>
 __global__ __launch_bounds__(kConvThreads) void conv2_kernel(Conv2Args a) {
     extern __shared__ u4v lds[];
+    constexpr int SOUT = SP - 2, S2 = SOUT * SOUT;
     constexpr int NTP = CIN == 16 ? kTP : 2;
+    constexpr int VPC = CIN == 16 ? 2 : 1;  // staged words per cell: 2 x 16 B, or one 4-byte pair
     const int tile = blockIdx.x / a.slabs, slab = blockIdx.x - tile * a.slabs;
-    const int z0 = slab * kZS, zn = min(kZS, a.sout - z0);
-    const int sin = a.sin, sout = a.sout, off = a.off;
-    // stage input planes z0 + off .. z0 + off + zn + 1 (16-byte vectors)
-    const int plane_v = sin * sin * CIN / 8;
+    const int z0 = slab * kZS, zn = min(kZS, SOUT - z0);
+    const int sin = a.sin, off = a.off;
     const int64_t tin = (int64_t)(a.idx ? a.idx[tile] : tile) * sin * sin * sin;
-    const u4v *src = reinterpret_cast<const u4v *>(a.in + tin * CIN);
-    for (int i = threadIdx.x; i < (zn + 2) * plane_v; i += kConvThreads) {
-        const int p = i / plane_v, r = i - p * plane_v;
-        const int iz = z0 + off + p;
-        u4v v = {0u, 0u, 0u, 0u};
-        if (iz >= 0 && iz < sin) v = src[(int64_t)iz * plane_v + r];
-        lds[i] = v;
+    {
+        // stage planes z0 + off .. z0 + off + zn + 1, rows and columns off .. off + SP - 1
+        const int nst = (zn + 2) * SP * SP * VPC;
+        for (int i = threadIdx.x; i < nst; i += kConvThreads) {
+            const unsigned c = (unsigned)i / VPC, v = (unsigned)i - c * VPC;
+            const int p = (int)(c / (SP * SP)), r = (int)(c - p * (SP * SP)), y = r / SP, x = r - y * SP;
+            const int iz = z0 + off + p, iy = off + y, ix = off + x;
+            const bool in = (unsigned)iz < (unsigned)sin && (unsigned)iy < (unsigned)sin && (unsigned)ix < (unsigned)sin;
+            const int64_t g = ((int64_t)iz * sin + iy) * sin + ix;
+            if constexpr (CIN == 16) {
+                u4v w = {0u, 0u, 0u, 0u};
+                if (in) w = reinterpret_cast<const u4v *>(a.in)[(tin + g) * 2 + v];
+                lds[i] = w;
+            } else {
+                uint32_t w = 0u;
+                if (in) w = reinterpret_cast<const uint32_t *>(a.in)[tin + g];
+                reinterpret_cast<uint32_t *>(lds)[i] = w;
+            }
+        }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     u4v wv[NTP];
 #pragma unroll
     for (int tp = 0; tp < NTP; ++tp) wv[tp] = a.wb[tp * 64 + lane];
+    const int m = lane & 15, kb = lane >> 4, h = kb >> 1;
+    // CIN 16: per-lane shifts of the three runs of tap pairs (16-byte words);
+    // CIN 2: staged-cell offsets of this lane's 8 taps 16 tp + 4 kb + j (past 26: any in-bounds cell)
+    const int dA = h * SP * SP * 2, dB = h * SP * 2, dC = h * 2;
+    int toff[CIN == 16 ? 1 : 8];
+    if constexpr (CIN != 16) {
+#pragma unroll
+        for (int tp = 0; tp < 2; ++tp)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int t = 16 * tp + 4 * kb + j;
+                toff[tp * 4 + j] = t < 27 ? (t / 9) * SP * SP + ((t / 3) % 3) * SP + t % 3 : 0;
+            }
+    }
     __syncthreads();
-    const int m = lane & 15, kb = lane >> 4;
-    const int s2 = sout * sout, ncell = zn * s2, groups = (ncell + 15) / 16;
+    const int ncell = zn * S2, groups = (ncell + 15) / 16;
     float bias[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) bias[j] = a.bias ? a.bias[4 * kb + j] : 0.0f;
     const uint32_t *L32 = reinterpret_cast<const uint32_t *>(lds);
     constexpr int kWaves = kConvThreads / 64;
     for (int g0 = wave * kCG; g0 < groups; g0 += kWaves * kCG) {
-        int cx[kCG], cy[kCG], cz[kCG];
+        int base[kCG];
 #pragma unroll
         for (int g = 0; g < kCG; ++g) {
-            int q = (g0 + g) * 16 + m;
-            if (q >= ncell) q = ncell - 1;  // past the last cell: computed, not stored
-            cz[g] = q / s2;
-            const int r = q - cz[g] * s2;
-            cy[g] = r / sout;
-            cx[g] = r - cy[g] * sout;
+            const unsigned q = (unsigned)min((g0 + g) * 16 + m, ncell - 1);  // past the last cell: computed, not stored
+            const unsigned cz = q / S2, r = q - cz * S2, cy = r / SOUT, cx = r - cy * SOUT;
+            base[g] = (int)((cz * SP + cy) * SP + cx);  // staged cell of tap (0, 0, 0)
+            if constexpr (CIN == 16) base[g] = base[g] * 2 + (kb & 1);
         }
         f4 acc[kCG];
 #pragma unroll
@@ -133,26 +169,14 @@ __global__ __launch_bounds__(kConvThreads) void conv2_kernel(Conv2Args a) {
 #pragma unroll
             for (int g = 0; g < kCG; ++g) {
                 if constexpr (CIN == 16) {
-                    const int t = 2 * tp + (kb >> 1);
-                    u4v v = {0u, 0u, 0u, 0u};
-                    if (t < 27) {
-                        const int ix = cx[g] + off + t % 3, iy = cy[g] + off + (t / 3) % 3, iz = cz[g] + t / 9;
-                        if ((unsigned)ix < (unsigned)sin && (unsigned)iy < (unsigned)sin)
-                            v = lds[((iz * sin + iy) * sin + ix) * 2 + (kb & 1)];
-                    }
-                    av[g] = v;
+                    const int t0 = tap_of(tp, 0);
+                    const int c0 = 2 * ((t0 / 9) * SP * SP + ((t0 / 3) % 3) * SP + t0 % 3);
+                    const int sh = tp < 9 ? dA : tp < 12 ? dB : tp == 12 ? dC : 0;
+                    av[g] = lds[base[g] + sh + c0];
                 } else {
-                    // taps 16 tp + 4 kb + j, j < 4, both channels (4 bytes each)
-                    u4v v = {0u, 0u, 0u, 0u};
+                    u4v v;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int t = 16 * tp + 4 * kb + j;
-                        if (t < 27) {
-                            const int ix = cx[g] + off + t % 3, iy = cy[g] + off + (t / 3) % 3, iz = cz[g] + t / 9;
-                            if ((unsigned)ix < (unsigned)sin && (unsigned)iy < (unsigned)sin)
-                                v[j] = L32[(iz * sin + iy) * sin + ix];
-                        }
-                    }
+                    for (int j = 0; j < 4; ++j) v[j] = L32[base[g] + toff[tp * 4 + j]];
                     av[g] = v;
                 }
             }
@@ -166,7 +190,7 @@ __global__ __launch_bounds__(kConvThreads) void conv2_kernel(Conv2Args a) {
             const int grp = g0 + g;
             const int cl = grp * 16 + m;  // this lane's cell; channels 4 kb .. 4 kb + 3
             if (grp >= groups || cl >= ncell) continue;
-            const int64_t o = ((int64_t)tile * sout + z0) * s2 + cl;
+            const int64_t o = ((int64_t)tile * SOUT + z0) * S2 + cl;
             float v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -211,7 +235,7 @@ __global__ void prep_wb_kernel(const float *__restrict__ params, u4v *wb, u4v *w
             c = e & 1;
             if (tp >= 2) t = 27;
         } else {
-            t = 2 * tp + (kb >> 1);
+            t = tap_of(tp, kb >> 1);
             c = (kb & 1) * 8 + e;
         }
         f[e] = t < 27 ? to_bf16(W[(t * 16 + c) * 16 + n]) : (uint16_t)0;
@@ -557,32 +581,34 @@ int ensure_ws(msbg_learned *n, int64_t tiles, bool train) {
     return MSBG_OK;
 }
 
-template <int CIN, bool OUTF>
-void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int sout, int off, const u4v *wb,
+// Output edge SOUT; the layer reads input cells off .. off + SOUT + 1 of each
+// sin^3 tile along every axis (zero outside the tile).
+template <int CIN, int SOUT, bool OUTF>
+void conv(hipStream_t s, const uint16_t *in, const int32_t *idx, int sin, int off, const u4v *wb,
           const float *bias, int relu, const uint16_t *mask, void *out, int n) {
     Conv2Args a;
     a.in = in;
     a.idx = idx;
     a.sin = sin;
-    a.sout = sout;
     a.off = off;
     a.wb = wb;
     a.bias = bias;
     a.relu = relu;
     a.mask = mask;
     a.out = out;
-    a.slabs = (sout + kZS - 1) / kZS;
-    const size_t lds = (size_t)(kZS + 2) * sin * sin * CIN * 2;
-    conv2_kernel<CIN, OUTF><<<n * a.slabs, kConvThreads, lds, s>>>(a);
+    a.slabs = (SOUT + kZS - 1) / kZS;
+    constexpr int SP = SOUT + 2;
+    const size_t lds = (size_t)(kZS + 2) * SP * SP * CIN * 2;
+    conv2_kernel<CIN, SP, OUTF><<<n * a.slabs, kConvThreads, lds, s>>>(a);
 }
 
 void forward(msbg_learned *net, hipStream_t s, const uint16_t *x0, const int32_t *idx, int n) {
     const u4v *W = net->d_wb;
     const float *P = net->d_params;
-    conv<2, false>(s, x0, idx, 20, 18, 0, W, P + 0 * kLP + kTap, 1, nullptr, net->A1, n);
-    conv<16, false>(s, net->A1, nullptr, 18, 16, 0, W + kWB, P + 1 * kLP + kTap, 1, nullptr, net->A2, n);
-    conv<16, false>(s, net->A2, nullptr, 16, 16, -1, W + 2 * kWB, P + 2 * kLP + kTap, 1, nullptr, net->A3, n);
-    conv<16, true>(s, net->A3, nullptr, 16, 16, -1, W + 3 * kWB, P + 3 * kLP + kTap, 0, nullptr, net->Y, n);
+    conv<2, 18, false>(s, x0, idx, 20, 0, W, P + 0 * kLP + kTap, 1, nullptr, net->A1, n);
+    conv<16, 16, false>(s, net->A1, nullptr, 18, 0, W + kWB, P + 1 * kLP + kTap, 1, nullptr, net->A2, n);
+    conv<16, 16, false>(s, net->A2, nullptr, 16, -1, W + 2 * kWB, P + 2 * kLP + kTap, 1, nullptr, net->A3, n);
+    conv<16, 16, true>(s, net->A3, nullptr, 16, -1, W + 3 * kWB, P + 3 * kLP + kTap, 0, nullptr, net->Y, n);
 }
 
 template <int CIN>
@@ -615,11 +641,11 @@ void loss_grad(msbg_learned *net, hipStream_t s, const uint16_t *x0, const float
     const u4v *Wf = net->d_wbf;
     float *g = net->d_grad;
     wgrad<16>(net, s, net->A3, nullptr, 16, 16, -1, net->G4, n, g + 3 * kLP);
-    conv<16, false>(s, net->G4, nullptr, 16, 16, -1, Wf + 3 * kWB, nullptr, 0, net->A3, net->G3, n);
+    conv<16, 16, false>(s, net->G4, nullptr, 16, -1, Wf + 3 * kWB, nullptr, 0, net->A3, net->G3, n);
     wgrad<16>(net, s, net->A2, nullptr, 16, 16, -1, net->G3, n, g + 2 * kLP);
-    conv<16, false>(s, net->G3, nullptr, 16, 16, -1, Wf + 2 * kWB, nullptr, 0, net->A2, net->G2, n);
+    conv<16, 16, false>(s, net->G3, nullptr, 16, -1, Wf + 2 * kWB, nullptr, 0, net->A2, net->G2, n);
     wgrad<16>(net, s, net->A1, nullptr, 18, 16, 0, net->G2, n, g + 1 * kLP);
-    conv<16, false>(s, net->G2, nullptr, 16, 18, -2, Wf + 1 * kWB, nullptr, 0, net->A1, net->G1, n);
+    conv<16, 18, false>(s, net->G2, nullptr, 16, -2, Wf + 1 * kWB, nullptr, 0, net->A1, net->G1, n);
     wgrad<2>(net, s, x0, idx, 20, 18, 0, net->G1, n, g + 0 * kLP);
 }
 
@@ -710,6 +736,10 @@ int msbg_learned_new(msbg_learned **out) {
     if (e == hipSuccess) e = hipMemset(n->d_params, 0, kParams * 4);
     if (e == hipSuccess) e = hipMemset(n->d_m, 0, kParams * 4);
     if (e == hipSuccess) e = hipMemset(n->d_v, 0, kParams * 4);
+    // the data gradient of layer 1 stages 6 planes of 20^2 x 16 channels (76.8 KB)
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)conv2_kernel<16, 20, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (kZS + 2) * 20 * 20 * 16 * 2);
     if (e != hipSuccess) {
         msbg_learned_free(n);
         return hip_fail(e, "msbg_learned_new");
