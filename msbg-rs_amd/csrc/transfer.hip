@@ -246,8 +246,21 @@ __device__ __forceinline__ float splat_wg_sum(float v, float *s_red) {
 constexpr int kResCap = 2048;
 constexpr int kSplatMaxCells = 8192;  // window cells of CH 2 (m <= 2: 20^3)
 
-template <int CH>
-__global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, GridView gv, int a,
+// |q| of an integral float, 2^24 <= |q| < 2^62, as a 64-bit integer:
+// |q| = t 2^32 + lo with t = trunc(|q| 2^-32) and 0 <= lo < 2^32, both exact
+// in float (below 2^32 lo is |q| itself; above, the ulp of |q| is >= 2^9, so
+// lo needs at most 23 bits).
+__device__ __forceinline__ unsigned long long integral_mag_u64(float q) {
+    const float a = fabsf(q);
+    const float t = truncf(a * 0x1p-32f);
+    const float lo = __builtin_fmaf(t, -0x1p32f, a);
+    return (unsigned long long)(uint32_t)t << 32 | (unsigned long long)(uint32_t)lo;
+}
+
+// AT is the face direction when it is a template constant (the one-pass
+// CH 2 kernels), or -1 when it comes in `a_rt`.
+template <int CH, int AT>
+__global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, GridView gv, int a_rt,
                                                                   const SplatItem *__restrict__ items,
                                                                   const int32_t *__restrict__ pblocks,
                                                                   const float *__restrict__ spos,
@@ -257,7 +270,9 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
     extern __shared__ __attribute__((aligned(8))) unsigned char smem_raw[];
     __shared__ float s_red[kSplatThreads / 64];
     __shared__ int s_nres;
+    __shared__ int s_slot[2][27];  // mass / velocity slot of each committed region, -1 where not committed
     constexpr int NC = CH == 2 ? 2 : 1;  // channels of this pass
+    const int a = AT >= 0 ? AT : a_rt;
     const int MSX = 16 + 2 * m, NS = MSX * MSX * MSX;
     unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem_raw);  // NC windows
     float *res = reinterpret_cast<float *>(acc + NC * NS);                         // CH 0 / 1: float window
@@ -272,6 +287,27 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
     const int b0x = bx * 16, b0y = by * 16, b0z = bz * 16;
     const int x0s = b0x - m, y0s = b0y - m, z0s = b0z - m, whi = 16 + m - 1;
     const float ofx = a == 0 ? 1.0f : 0.5f, ofy = a == 1 ? 1.0f : 0.5f, ofz = a == 2 ? 1.0f : 0.5f;
+    if (threadIdx.x < 27) {
+        // Commit regions (p2g.rs:342-411): window region (rx, ry, rz) is the
+        // margin below, the block's own 16 cells or the margin above on each
+        // axis; it is committed when it lies inside the domain and its block
+        // holds both a mass and a velocity slot.
+        const int t = threadIdx.x, rx = t % 3, ry = (t / 3) % 3, rz = t / 9;
+        const int xlo = rx == 0 ? 0 : (rx == 1 ? m : m + 16), xlen = rx == 1 ? 16 : m;
+        const int ylo = ry == 0 ? 0 : (ry == 1 ? m : m + 16), ylen = ry == 1 ? 16 : m;
+        const int zlo = rz == 0 ? 0 : (rz == 1 ? m : m + 16), zlen = rz == 1 ? 16 : m;
+        const int gx0 = x0s + xlo, gy0 = y0s + ylo, gz0 = z0s + zlo;
+        int ms = -1, vs = -1;
+        if (gx0 >= 0 && gy0 >= 0 && gz0 >= 0 && gx0 + xlen <= gm.sx && gy0 + ylen <= gm.sy && gz0 + zlen <= gm.sz) {
+            const int64_t nb = (int64_t)(bx + rx - 1) + (int64_t)(by + ry - 1) * gm.nx +
+                               (int64_t)(bz + rz - 1) * gm.nx * gm.ny;
+            ms = gm.b2s[nb];
+            vs = gv.b2s[nb];
+            if (ms < 2 || vs < 2) ms = vs = -1;
+        }
+        s_slot[0][t] = ms;
+        s_slot[1][t] = vs;
+    }
     // Lane t takes a contiguous run of the item's particles.
     const int p_beg = item.p0, p_cnt = item.p1 - item.p0;
     const int per = (p_cnt + (int)blockDim.x - 1) / (int)blockDim.x;
@@ -307,9 +343,17 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
         const float bound = splat_wg_sum(bsum[k], s_red);
         any = any || bound > 0.0f;
         frexpf(bound > 0.0f && isfinite(bound) ? bound : 1.0f, &ex[k]);  // bound < 2^e
-        up[k] = ldexpf(1.0f, 61 - ex[k]);                                 // 2^S
+        // S = 61 - e, at most 126: 2^S stays a float, and a contribution
+        // reaching 2^24 after scaling comes from a normal float
+        ex[k] = max(ex[k], 61 - 126);
+        up[k] = ldexpf(1.0f, 61 - ex[k]);  // 2^S
     }
     const bool item_nonfin = splat_wg_sum(nonfin ? 1.0f : 0.0f, s_red) > 0.0f;  // uniform per group
+    // S >= 0 on every channel: a finite nonzero scale times 2^S is nonzero,
+    // so a zero scaled contribution is a zero contribution
+    bool unit_scales = true;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) unit_scales = unit_scales && ex[k] <= 61;
     // Nothing but zero contributions: the item adds nothing.  Uniform per group.
     if (!any && !item_nonfin) return;
     constexpr float kExact = 16777216.0f;  // 2^24
@@ -331,50 +375,110 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
         const float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
         return w * w * w;
     };
+    // One contribution w * sc[k] of a particle whose scales are finite (sf =
+    // sc, sq = sc 2^S; both 0 for a non-finite channel).  q = w sq equals
+    // (w sc) 2^S bit for bit whenever |q| >= 2^24 (2^S is a power of two and
+    // w sc is then a normal float), so the window receives the same integers
+    // as from scaling the float contribution.
+    auto contrib = [&](int cell, float w, const float *sf, const float *sq) {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const float q = w * sq[k];
+            if (fabsf(q) >= kExact) {
+                const unsigned long long u = integral_mag_u64(q);
+                if (q > 0.0f)
+                    atomicAdd(&acc[k * NS + cell], u);
+                else
+                    __hip_atomic_fetch_sub(&acc[k * NS + cell], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                const float c = w * sf[k];
+                if (c == 0.0f) continue;  // +-0 keeps the value
+                if (CH != 2) {
+                    atomicAdd(&res[cell], c);
+                } else {
+                    const int slot = atomicAdd(&s_nres, 1);
+                    if (slot < kResCap)
+                        rlist[slot] = make_uint2((uint32_t)cell | (uint32_t)k << 31, __float_as_uint(c));
+                    else
+                        atomicAdd(&acc[k * NS + cell], (unsigned long long)llrintf(c * up[k]));
+                }
+            }
+        }
+    };
     for (int j = 0; j < per; ++j) {
         const int pi = p_beg + (int)threadIdx.x * per + j;
         if (pi >= p_beg + p_cnt) break;
-        const float px = spos[3 * pi], py = spos[3 * pi + 1], pz = spos[3 * pi + 2];
-        float sc[NC];
+        float sf[NC], sq[NC];
+        unsigned long long sgn[NC];  // all ones where the channel's contributions are negative
+        bool live = false;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) sc[k] = scale_of(pi, k);
-        const float px0 = px - 0.5f, py0 = py - 0.5f, pz0 = pz - 0.5f;
+        for (int k = 0; k < NC; ++k) {
+            const float v = scale_of(pi, k);
+            const bool fin = isfinite(v);  // non-finite: added after the commit
+            sf[k] = fin ? v : 0.0f;
+            sq[k] = fin ? v * up[k] : 0.0f;
+            sgn[k] = sq[k] < 0.0f ? ~0ull : 0ull;
+            live = live || (fin && v != 0.0f);
+        }
+        if (!live) continue;
+        const float px0 = spos[3 * pi] - 0.5f, py0 = spos[3 * pi + 1] - 0.5f, pz0 = spos[3 * pi + 2] - 0.5f;
         int ix1, ix2, iy1, iy2, iz1, iz2;
         footprint(pi, ix1, ix2, iy1, iy2, iz1, iz2);
-        for (int iz = iz1; iz <= iz2; ++iz) {
-            const float dz = (float)iz - pz0;
-            const float dz2 = dz * dz;
-            for (int iy = iy1; iy <= iy2; ++iy) {
-                const float dy = (float)iy - py0;
-                const float dyz = dz2 + dy * dy;
-                const int row = ((iz - z0s) * MSX + (iy - y0s)) * MSX;
-                for (int ix = ix1; ix <= ix2; ++ix) {
-                    const float dx = (float)ix - px0;
-                    const float dist_sq = dx * dx + dyz;
-                    const float d_a = a == 0 ? dx : (a == 1 ? dy : dz);
-                    const float df2 = dist_sq + d_a + 0.25f;
-                    float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
-                    w = w * w * w;
-                    const int cell = row + ix - x0s;
+        if (AT >= 0 && unit_scales && ix2 - ix1 <= 3 && iy2 - iy1 <= 3 && iz2 - iz1 <= 3) {
+            // r_p <= 2: the footprint is at most 4 cells wide on an axis
+            // unless p - of is an integer there (5 wide: the loop below).
+            // The 4^3 box is unrolled; offsets past the footprint get an
+            // infinite squared distance, hence weight 0 and no LDS access.
+            float cx[4], cx2[4], cy[4], cy2[4], cz[4], cz2[4];
 #pragma unroll
-                    for (int k = 0; k < NC; ++k) {
-                        const float c = w * sc[k];
-                        if (c == 0.0f || !isfinite(sc[k])) continue;  // +-0 keeps the value; non-finite: below
-                        const float q = c * up[k];  // exact: a power-of-two scale within range
-                        if (fabsf(q) >= kExact) {
-                            atomicAdd(&acc[k * NS + cell], (unsigned long long)(long long)q);
-                        } else if (CH != 2) {
-                            atomicAdd(&res[cell], c);
+            for (int o = 0; o < 4; ++o) {
+                cx[o] = (float)(ix1 + o) - px0;
+                cy[o] = (float)(iy1 + o) - py0;
+                cz[o] = (float)(iz1 + o) - pz0;
+                cx2[o] = ix1 + o <= ix2 ? cx[o] * cx[o] : __builtin_inff();
+                cy2[o] = iy1 + o <= iy2 ? cy[o] * cy[o] : __builtin_inff();
+                cz2[o] = iz1 + o <= iz2 ? cz[o] * cz[o] : __builtin_inff();
+            }
+            const int base = ((iz1 - z0s) * MSX + (iy1 - y0s)) * MSX + ix1 - x0s;
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz)
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const float dyz = cz2[oz] + cy2[oy];
+                    const int row = base + (oz * MSX + oy) * MSX;
+#pragma unroll
+                    for (int ox = 0; ox < 4; ++ox) {
+                        const float dist_sq = cx2[ox] + dyz;
+                        const float d_a = AT == 0 ? cx[ox] : (AT == 1 ? cy[oy] : cz[oz]);
+                        const float df2 = dist_sq + d_a + 0.25f;
+                        float w = fmaxf(1.0f - df2 * inv_r2, 0.0f);
+                        w = w * w * w;
+                        // Common case, no branch: every lane's scaled
+                        // contribution is 0 or at least 2^24 in magnitude,
+                        // so its magnitude converts exactly and the signed
+                        // add (zero included) goes straight to the window.
+                        float q[NC];
+                        bool odd = false;
+#pragma unroll
+                        for (int k = 0; k < NC; ++k) {
+                            q[k] = w * sq[k];
+                            odd = odd || (fabsf(q[k]) < kExact && q[k] != 0.0f);
+                        }
+                        if (__any(odd)) {
+                            contrib(row + ox, w, sf, sq);
                         } else {
-                            const int slot = atomicAdd(&s_nres, 1);
-                            if (slot < kResCap)
-                                rlist[slot] = make_uint2((uint32_t)cell | (uint32_t)k << 31, __float_as_uint(c));
-                            else
-                                atomicAdd(&acc[k * NS + cell], (unsigned long long)llrintf(q));
+#pragma unroll
+                            for (int k = 0; k < NC; ++k)
+                                atomicAdd(&acc[k * NS + row + ox], (integral_mag_u64(q[k]) ^ sgn[k]) - sgn[k]);
                         }
                     }
                 }
-            }
+        } else {
+            for (int iz = iz1; iz <= iz2; ++iz)
+                for (int iy = iy1; iy <= iy2; ++iy)
+                    for (int ix = ix1; ix <= ix2; ++ix)
+                        contrib(((iz - z0s) * MSX + (iy - y0s)) * MSX + ix - x0s, weight(px0, py0, pz0, ix, iy, iz),
+                                sf, sq);
         }
     }
     __syncthreads();
@@ -419,37 +523,52 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
     }
     __syncthreads();
     // Commit (p2g.rs:342-411): staging cell -> its region -> the region's
-    // block.  The target of staging cell s for channel k, or null where the
-    // region is not committed.
+    // block (the region table above).  Window cell s = (lz MSX + ly) MSX + lx
+    // is split in float: (s + 1/2) / n is at least 1/(2n) away from an
+    // integer, far beyond its rounding.
+    const float inv_m1 = 1.0f / (float)MSX, inv_m2 = 1.0f / (float)(MSX * MSX);
+    auto region = [&](int l) { return l < m ? 0 : (l < m + 16 ? 1 : 2); };
+    // The target of staging cell s for channel k, or null where its region is not committed.
     auto target = [&](int s, int k) -> float * {
-        const int lx = s % MSX, ly = (s / MSX) % MSX, lz = s / (MSX * MSX);
-        const int dx = lx < m ? -1 : (lx < m + 16 ? 0 : 1);
-        const int dy = ly < m ? -1 : (ly < m + 16 ? 0 : 1);
-        const int dz = lz < m ? -1 : (lz < m + 16 ? 0 : 1);
-        const int xlo = dx < 0 ? 0 : (dx == 0 ? m : m + 16), xlen = dx == 0 ? 16 : m;
-        const int ylo = dy < 0 ? 0 : (dy == 0 ? m : m + 16), ylen = dy == 0 ? 16 : m;
-        const int zlo = dz < 0 ? 0 : (dz == 0 ? m : m + 16), zlen = dz == 0 ? 16 : m;
-        const int gx0 = b0x + xlo - m, gy0 = b0y + ylo - m, gz0 = b0z + zlo - m;
-        if (gx0 < 0 || gy0 < 0 || gz0 < 0) return nullptr;
-        if (gx0 + xlen > gm.sx || gy0 + ylen > gm.sy || gz0 + zlen > gm.sz) return nullptr;
-        const int64_t nb = (int64_t)(bx + dx) + (int64_t)(by + dy) * gm.nx + (int64_t)(bz + dz) * gm.nx * gm.ny;
-        const int gx = x0s + lx, gy = y0s + ly, gz = z0s + lz;
-        const int vid = (gx & 15) | ((gy & 15) << 4) | ((gz & 15) << 8);
-        const int ms = gm.b2s[nb], vs = gv.b2s[nb];
-        if (ms < 2 || vs < 2) return nullptr;
-        return is_mass(k) ? gm.slab + (int64_t)ms * 4096 + vid : gv.slab + (int64_t)vs * 4096 + vid;
+        const int lz = (int)(((float)s + 0.5f) * inv_m2), r = s - lz * MSX * MSX;
+        const int ly = (int)(((float)r + 0.5f) * inv_m1), lx = r - ly * MSX;
+        const int slot = s_slot[is_mass(k) ? 0 : 1][region(lx) + 3 * region(ly) + 9 * region(lz)];
+        if (slot < 0) return nullptr;
+        const int vid = ((x0s + lx) & 15) | (((y0s + ly) & 15) << 4) | (((z0s + lz) & 15) << 8);
+        return (is_mass(k) ? gm.slab : gv.slab) + (int64_t)slot * 4096 + vid;
     };
-    for (int s = threadIdx.x; s < NS; s += blockDim.x) {
+    // Four window cells per lane at a time, their reads issued before their writes.
+    for (int s0 = threadIdx.x; s0 < NS; s0 += 4 * kSplatThreads) {
+        float *pt[4][NC];
+        float val[4][NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            const float val = fv[k][s];
-            if (val == 0.0f) continue;
-            float *po = target(s, k);
-            if (!po) continue;
-            if (item.shared)  // blocks of one colour never share a window: only the block's own items race
-                unsafeAtomicAdd(po, val);
-            else
-                *po += val;
+        for (int i = 0; i < 4; ++i) {
+            const int s = s0 + i * kSplatThreads;
+            const bool in = s < NS;
+            const int sc = in ? s : 0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                val[i][k] = in ? fv[k][sc] : 0.0f;
+                pt[i][k] = val[i][k] != 0.0f ? target(sc, k) : nullptr;
+            }
+        }
+        if (item.shared) {  // blocks of one colour never share a window: only the block's own items race
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                    if (pt[i][k]) unsafeAtomicAdd(pt[i][k], val[i][k]);
+        } else {
+            float old[4][NC];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) old[i][k] = pt[i][k] ? *pt[i][k] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                    if (pt[i][k]) *pt[i][k] = old[i][k] + val[i][k];
         }
     }
     if (!item_nonfin) return;
@@ -1000,12 +1119,11 @@ int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, co
         MSBG_CHECK((int64_t)lds <= max_lds, MSBG_EINVAL,
                    "particles_to_grid: the staging window for this r_p exceeds the workgroup LDS limit");
         if (lds > 65536) {
-            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<0>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<1>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            MSBG_HIP(hipFuncSetAttribute((const void *)mac_splat_kernel<2>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const void *fns[5] = {(const void *)mac_splat_kernel<0, -1>, (const void *)mac_splat_kernel<1, -1>,
+                                  (const void *)mac_splat_kernel<2, 0>, (const void *)mac_splat_kernel<2, 1>,
+                                  (const void *)mac_splat_kernel<2, 2>};
+            for (const void *f : fns)
+                MSBG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
     }
     const float inv_r2 = 1.0f / (r_p * r_p);
@@ -1013,16 +1131,24 @@ int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, co
         int64_t off = 0;
         for (int c = 0; c < 8; ++c) {
             if (cnt[c] && both) {
-                hipLaunchKernelGGL(mac_splat_kernel<2>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
-                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
-                                   inv_r2, m);
+                const dim3 grid((unsigned)cnt[c]), block(kSplatThreads);
+                const GridView mv = mass[a]->view(), vv = vel[a]->view();
+                if (a == 0)
+                    mac_splat_kernel<2, 0><<<grid, block, lds, stream()>>>(mv, vv, a, d_order + off, pblocks, spos,
+                                                                          svel, smass, r_p, inv_r2, m);
+                else if (a == 1)
+                    mac_splat_kernel<2, 1><<<grid, block, lds, stream()>>>(mv, vv, a, d_order + off, pblocks, spos,
+                                                                          svel, smass, r_p, inv_r2, m);
+                else
+                    mac_splat_kernel<2, 2><<<grid, block, lds, stream()>>>(mv, vv, a, d_order + off, pblocks, spos,
+                                                                          svel, smass, r_p, inv_r2, m);
             } else if (cnt[c]) {
-                hipLaunchKernelGGL(mac_splat_kernel<0>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
-                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
-                                   inv_r2, m);
-                hipLaunchKernelGGL(mac_splat_kernel<1>, dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds, stream(),
-                                   mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel, smass, r_p,
-                                   inv_r2, m);
+                hipLaunchKernelGGL((mac_splat_kernel<0, -1>), dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds,
+                                   stream(), mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel,
+                                   smass, r_p, inv_r2, m);
+                hipLaunchKernelGGL((mac_splat_kernel<1, -1>), dim3((unsigned)cnt[c]), dim3(kSplatThreads), lds,
+                                   stream(), mass[a]->view(), vel[a]->view(), a, d_order + off, pblocks, spos, svel,
+                                   smass, r_p, inv_r2, m);
             }
             off += cnt[c];
         }
