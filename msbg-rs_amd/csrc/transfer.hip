@@ -438,6 +438,9 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
                 cx2[o] = ix1 + o <= ix2 ? cx[o] * cx[o] : __builtin_inff();
                 cy2[o] = iy1 + o <= iy2 ? cy[o] * cy[o] : __builtin_inff();
                 cz2[o] = iz1 + o <= iz2 ? cz[o] * cz[o] : __builtin_inff();
+                // computed once per particle: the empty statements keep the
+                // compiler from recomputing them at each of the 64 cells
+                asm volatile("" : "+v"(cx[o]), "+v"(cx2[o]));
             }
             const int base = ((iz1 - z0s) * MSX + (iy1 - y0s)) * MSX + ix1 - x0s;
 #pragma unroll
@@ -462,7 +465,7 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
 #pragma unroll
                         for (int k = 0; k < NC; ++k) {
                             q[k] = w * sq[k];
-                            odd = odd || (fabsf(q[k]) < kExact && q[k] != 0.0f);
+                            odd = odd | ((fabsf(q[k]) < kExact) & (q[k] != 0.0f));  // no short circuit
                         }
                         if (__any(odd)) {
                             contrib(row + ox, w, sf, sq);
