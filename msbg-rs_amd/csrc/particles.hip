@@ -18,6 +18,22 @@ __global__ void morton_keys_kernel(const float *__restrict__ pos, int64_t n, uin
     }
 }
 
+// morton_keys_kernel's keys and indices, and each particle's two 16-byte
+// records (x, y, z, m) and (u, v, w, 0) in caller order, so that the sorted
+// payload is gathered as one 32-byte record per particle instead of three
+// scattered reads.
+__global__ void morton_keys_pack_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
+                                        const float *__restrict__ mass, int64_t n, uint32_t *__restrict__ keys,
+                                        uint32_t *__restrict__ idx, float4 *__restrict__ rec) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        keys[i] = spread10(trunc_index(px)) | (spread10(trunc_index(py)) << 1) | (spread10(trunc_index(pz)) << 2);
+        idx[i] = (uint32_t)i;
+        rec[2 * i] = make_float4(px, py, pz, mass ? mass[i] : 1.0f);
+        rec[2 * i + 1] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f);
+    }
+}
+
 // Row-major block id of a particle (p2g.rs:149-157).
 __global__ void particle_bids_kernel(const float *__restrict__ pos, int64_t n, int32_t nx, int32_t ny,
                                      uint32_t *__restrict__ bids, uint32_t *__restrict__ idx) {
@@ -171,6 +187,15 @@ int noalloc_scratch(uint32_t **out) {
 int morton_sort_async(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm) {
     if (n == 0) return MSBG_OK;
     hipLaunchKernelGGL(morton_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, n, d_keys, d_perm);
+    MSBG_HIP(hipGetLastError());
+    return radix_sort_pairs(d_keys, d_perm, n, 30);
+}
+
+int morton_sort_pack_async(const float *d_pos, const float *d_vel, const float *d_mass, int64_t n, uint32_t *d_keys,
+                           uint32_t *d_perm, float4 *d_rec) {
+    if (n == 0) return MSBG_OK;
+    hipLaunchKernelGGL(morton_keys_pack_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), d_pos, d_vel, d_mass, n,
+                       d_keys, d_perm, d_rec);
     MSBG_HIP(hipGetLastError());
     return radix_sort_pairs(d_keys, d_perm, n, 30);
 }

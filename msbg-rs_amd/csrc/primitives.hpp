@@ -24,6 +24,10 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, int64_t n, int key_bits);
 
 // msbg_morton_sort without the final synchronization (stream-ordered).
 int morton_sort_async(const float *d_pos, int64_t n, uint32_t *d_keys, uint32_t *d_perm);
+// morton_sort_async, also writing each particle's records (x, y, z, m), (u, v, w, 0)
+// to d_rec[2 i], d_rec[2 i + 1] in caller order (d_mass null: m = 1).
+int morton_sort_pack_async(const float *d_pos, const float *d_vel, const float *d_mass, int64_t n, uint32_t *d_keys,
+                           uint32_t *d_perm, float4 *d_rec);
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {  // multires/sort.rs:8-16, 10 bits
     v &= 0x3ffu;

@@ -52,6 +52,12 @@ struct Row {
     __device__ __forceinline__ void store(typename Sto<ST>::T *p, i64 i) const {
         *reinterpret_cast<uint4 *>(p + i) = make_uint4(w[0], w[1], w[2], w[3]);
     }
+    // Streaming store: the row is not read again by this launch.
+    __device__ __forceinline__ void store_nt(typename Sto<ST>::T *p, i64 i) const {
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 v = {w[0], w[1], w[2], w[3]};
+        __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(p + i));
+    }
     __device__ __forceinline__ typename Sto<ST>::T bits(int i) const {
         if constexpr (ST == MSBG_STORAGE_F32)
             return __uint_as_float(w[i]);
@@ -182,8 +188,10 @@ __global__ __launch_bounds__(kThreads) void relax_kernel(const PBlock *__restric
             if (i < Q.n && (!RB || ((par + i) & 1) == color))
                 v.set(i, __builtin_fmaf(w6, col_residual<ST, Z>(Q, d, i, j, Q.b[j].at(i)), Q.c[j + 1].at(i)));
         const i64 cj = Q.c0 + 256 * j;
-        if (RB || Q.n == W) {
+        if (RB) {
             v.store(xout, cj);
+        } else if (Q.n == W) {
+            v.store_nt(xout, cj);
         } else {
 #pragma unroll
             for (int i = 0; i < W; ++i)

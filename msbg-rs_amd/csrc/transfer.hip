@@ -637,14 +637,14 @@ __device__ __forceinline__ float bspline_w(float p, int ci) {
     return 0.0f;
 }
 
-// Sorted particle records for the B-spline P2G: pm = (x, y, z, m), pu = (u, v, w, 0).
-__global__ void gather_payload4_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
-                                       const float *__restrict__ mass, const uint32_t *__restrict__ perm, int64_t n,
-                                       float4 *__restrict__ pm, float4 *__restrict__ pu) {
+// Sorted particle records for the B-spline P2G, pm = (x, y, z, m) and
+// pu = (u, v, w, 0), gathered from the caller-order records rec.
+__global__ void gather_records_kernel(const float4 *__restrict__ rec, const uint32_t *__restrict__ perm, int64_t n,
+                                      float4 *__restrict__ pm, float4 *__restrict__ pu) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = perm[i];
-        pm[i] = make_float4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass ? mass[j] : 1.0f);
-        pu[i] = make_float4(vel[3 * j], vel[3 * j + 1], vel[3 * j + 2], 0.0f);
+        pm[i] = rec[2 * j];
+        pu[i] = rec[2 * j + 1];
     }
 }
 
@@ -746,18 +746,26 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
                 j = s_rng[t][threadIdx.x].x;
                 e = s_rng[t][threadIdx.x].y;
             }
+            float4 pn = make_float4(0.f, 0.f, 0.f, 0.f), un = pn;
+            if (t < 9) {
+                pn = pm[j];
+                un = pu[j];
+            }
             while (t < 9) {
-                const float4 p = pm[j];
-                const float4 u = pu[j];
-                const float wxy = bspline_w<ORDER>(p.x, cx) * bspline_w<ORDER>(p.y, cy);
-                if (lv >= 1) acc_add<ORDER>(lo, wxy, p.z, cz0 + lv - 1, p.w, u);
-                if (lv >= 0 && lv <= 15) acc_add<ORDER>(mid, wxy, p.z, cz0 + lv, p.w, u);
-                if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
+                const float4 p = pn, u = un;
                 ++j;
                 while (j >= e && ++t < 9) {
                     j = s_rng[t][threadIdx.x].x;
                     e = s_rng[t][threadIdx.x].y;
                 }
+                if (t < 9) {  // the next particle's records are in flight while this one is added
+                    pn = pm[j];
+                    un = pu[j];
+                }
+                const float wxy = bspline_w<ORDER>(p.x, cx) * bspline_w<ORDER>(p.y, cy);
+                if (lv >= 1) acc_add<ORDER>(lo, wxy, p.z, cz0 + lv - 1, p.w, u);
+                if (lv >= 0 && lv <= 15) acc_add<ORDER>(mid, wxy, p.z, cz0 + lv, p.w, u);
+                if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
             }
         }
         if (lv >= 1) store(lv - 1, lo);  // cell lv-1 has its planes lv-2, lv-1, lv
@@ -1190,13 +1198,14 @@ int msbg_p2g_bspline(int order, msbg_grid *mass, msbg_grid *const *vel, const fl
     uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n, &rc);
     if (rc) return rc;
     uint32_t *perm = keys + n;
-    rc = morton_sort_async(d_pos, n, keys, perm);
+    float4 *rec = (float4 *)workspace(22, sizeof(float4) * 2 * n, &rc);
+    if (rc) return rc;
+    rc = morton_sort_pack_async(d_pos, d_vel, d_mass, n, keys, perm, rec);
     if (rc) return rc;
     float4 *pay = (float4 *)workspace(7, sizeof(float4) * 2 * n, &rc);
     if (rc) return rc;
     float4 *pm = pay, *pu = pay + n;
-    hipLaunchKernelGGL(gather_payload4_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass,
-                       perm, n, pm, pu);
+    hipLaunchKernelGGL(gather_records_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), rec, perm, n, pm, pu);
     MSBG_HIP(hipGetLastError());
     // per-voxel ranges of the sorted particles, indexed like the mass slab
     const int64_t ncell = mass->n_slots * 4096;
