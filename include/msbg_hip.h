@@ -289,6 +289,16 @@ int msbg_active_blocks_noalloc(int64_t sx, int64_t sy, int64_t sz, const float *
 /* d_dst[i] = d_src[d_perm[i]] for rows of row_bytes (a multiple of 4): the
  * payload gather of bucket_particles (fluid/mod.rs:84-89). */
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst);
+/* msbg_morton_sort followed by the payload gathers of positions (n x 3),
+ * velocities (n x 3) and masses (n; d_mass null: all 1) into the *_out
+ * arrays in sorted order -- the particle set bucket_by_block returns
+ * (particles/sort.rs:274-285) with the payload gather of fluid/mod.rs:84-89.
+ * Same keys, permutation and gathered values as msbg_morton_sort plus three
+ * msbg_gather_rows calls; the payload travels as one 32-byte record per
+ * particle, written while the keys are computed. */
+int msbg_morton_sort_particles(const float *d_pos, const float *d_vel, const float *d_mass, int64_t n,
+                               uint32_t *d_keys, uint32_t *d_perm, float *d_pos_out, float *d_vel_out,
+                               float *d_mass_out);
 
 /* ---- grid <-> particle transfers --------------------------------------------- */
 /* Sampler::sample -- math/sample.rs:489-526, with gather_map

@@ -34,6 +34,23 @@ __global__ void morton_keys_pack_kernel(const float *__restrict__ pos, const flo
     }
 }
 
+// Sorted particle arrays from the packed records: position, velocity and
+// mass of particle perm[i], one 32-byte record read per particle.
+__global__ void gather_particles_kernel(const float4 *__restrict__ rec, const uint32_t *__restrict__ perm, int64_t n,
+                                        float *__restrict__ pos, float *__restrict__ vel, float *__restrict__ mass) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = perm[i];
+        const float4 a = rec[2 * j], b = rec[2 * j + 1];
+        pos[3 * i] = a.x;
+        pos[3 * i + 1] = a.y;
+        pos[3 * i + 2] = a.z;
+        vel[3 * i] = b.x;
+        vel[3 * i + 1] = b.y;
+        vel[3 * i + 2] = b.z;
+        mass[i] = a.w;
+    }
+}
+
 // Row-major block id of a particle (p2g.rs:149-157).
 __global__ void particle_bids_kernel(const float *__restrict__ pos, int64_t n, int32_t nx, int32_t ny,
                                      uint32_t *__restrict__ bids, uint32_t *__restrict__ idx) {
@@ -307,6 +324,25 @@ int msbg_active_blocks_noalloc(int64_t sx, int64_t sy, int64_t sz, const float *
     MSBG_CHECK(scan_sums_words(nb) <= kNoAllocSumWords, MSBG_EINVAL, "active_blocks_noalloc: scan sums overflow");
     return active_blocks_into(d_pos, n, r_scan, (int32_t)nx, (int32_t)ny, (int32_t)nz, base, base + kNoAllocMaxBlocks,
                               base + 2 * kNoAllocMaxBlocks, d_out, n_out);
+}
+
+int msbg_morton_sort_particles(const float *d_pos, const float *d_vel, const float *d_mass, int64_t n,
+                               uint32_t *d_keys, uint32_t *d_perm, float *d_pos_out, float *d_vel_out,
+                               float *d_mass_out) {
+    MSBG_CHECK(n >= 0 && (n == 0 || (d_pos && d_vel && d_keys && d_perm && d_pos_out && d_vel_out && d_mass_out)),
+               MSBG_EINVAL, "morton sort particles: null argument");
+    MSBG_CHECK(n < ((int64_t)1 << 32), MSBG_EINVAL, "morton sort particles: more than 2^32 particles");
+    if (n == 0) return MSBG_OK;
+    int rc = 0;
+    float4 *rec = (float4 *)workspace(22, sizeof(float4) * 2 * (size_t)n, &rc);
+    if (rc) return rc;
+    rc = morton_sort_pack_async(d_pos, d_vel, d_mass, n, d_keys, d_perm, rec);
+    if (rc) return rc;
+    hipLaunchKernelGGL(gather_particles_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), rec, d_perm, n, d_pos_out,
+                       d_vel_out, d_mass_out);
+    MSBG_HIP(hipGetLastError());
+    MSBG_HIP(hipStreamSynchronize(stream()));
+    return MSBG_OK;
 }
 
 int msbg_gather_rows(const void *d_src, const uint32_t *d_perm, int64_t n, int64_t row_bytes, void *d_dst) {

@@ -4,7 +4,7 @@ The reference has the pieces up to the divergence (fluid/{mod,mac,p2g,g2p}.rs)
 and plans the pressure solve (docs/roadmap.md:610-702); this step composes
 them with the band Poisson solve:
 
-    Morton sort    msbg_morton_sort, msbg_gather_rows (positions, velocities, masses)
+    Morton sort    msbg_morton_sort_particles (keys, permutation, sorted positions, velocities, masses)
     P2G            particles_to_grid (fluid/p2g.rs:141-164: the MAC sum-splat, P/M normalize)
     copy           the pre-projection faces u* (msbg_grid_copy)
     divergence     MacGrid::divergence (fluid/mac.rs:137-171) over the MAC value blocks
@@ -56,11 +56,9 @@ class FlipStep:
         rec = lambda i: lib.call("msbg_event_record", ev[i])
         rec(0)
         keys, perm = lib.empty(n, np.uint32), lib.empty(n, np.uint32)
-        lib.call("msbg_morton_sort", pos.ptr, n, keys.ptr, perm.ptr)
         ps, vs, ms = lib.empty((n, 3), np.float32), lib.empty((n, 3), np.float32), lib.empty(n, np.float32)
-        lib.call("msbg_gather_rows", pos.ptr, perm.ptr, n, 12, ps.ptr)
-        lib.call("msbg_gather_rows", vel.ptr, perm.ptr, n, 12, vs.ptr)
-        lib.call("msbg_gather_rows", mass.ptr, perm.ptr, n, 4, ms.ptr)
+        lib.call("msbg_morton_sort_particles", pos.ptr, vel.ptr, mass.ptr, n, keys.ptr, perm.ptr, ps.ptr, vs.ptr,
+                 ms.ptr)
         rec(1)
         k = particles_to_grid(self.mac, ps, vs, ms, r_p)
         rec(2)

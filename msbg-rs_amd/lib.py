@@ -76,6 +76,7 @@ _SIGS = {
     "msbg_active_blocks": [i64, i64, i64, vp, i64, f32, vp, C.POINTER(C.c_int64)],
     "msbg_active_blocks_noalloc": [i64, i64, i64, vp, i64, f32, vp, C.POINTER(C.c_int64)],
     "msbg_gather_rows": [vp, vp, i64, i64, vp],
+    "msbg_morton_sort_particles": [vp, vp, vp, i64, vp, vp, vp, vp, vp],
     "msbg_sample": [vp, vp, i64, i32, i32, i32, vp],
     "msbg_mac_sample": [vp, vp, i64, i32, i32, vp],
     "msbg_g2p_flip": [vp, vp, vp, vp, i64, i32, i32, f32, vp],

@@ -59,6 +59,21 @@ def morton_sort(pos, lib=None):
     return keys, perm
 
 
+def morton_sort_particles(pos, vel, mass=None, lib=None):
+    """Morton-sorted particle set: (keys, perm, positions, velocities,
+    masses) as device arrays, the arrays in sorted order (one packed record
+    gathered per particle; the same values as morton_sort plus gather_rows)."""
+    lib = lib or load()
+    p, v = _dev(lib, pos, np.float32), _dev(lib, vel, np.float32)
+    m = _dev(lib, mass, np.float32) if mass is not None else None
+    n = p.shape[0]
+    keys, perm = lib.empty(n, np.uint32), lib.empty(n, np.uint32)
+    ps, vs, ms = lib.empty((n, 3), np.float32), lib.empty((n, 3), np.float32), lib.empty(n, np.float32)
+    lib.call("msbg_morton_sort_particles", p.ptr, v.ptr, m.ptr if m is not None else None, n, keys.ptr, perm.ptr,
+             ps.ptr, vs.ptr, ms.ptr)
+    return keys, perm, ps, vs, ms
+
+
 def particle_bids(sx, sy, sz, pos, lib=None):
     lib = lib or load()
     p = _dev(lib, pos, np.float32)

@@ -56,6 +56,26 @@ def test_morton_sort_matches_oracle(hip, orc):
     assert np.array_equal(perm.get(), want) and np.array_equal(keys.get(), k[want])
 
 
+@pytest.mark.parametrize("n,with_mass", [(0, True), (1, True), (4097, False), (300_000, True)])
+def test_morton_sort_particles_matches_sort_and_gathers(hip, orc, n, with_mass):
+    """The fused sort: keys and permutation as the oracle's stable Morton
+    sort, positions / velocities / masses the rows of that permutation (bit
+    for bit; masses 1 when none are given)."""
+    rng = np.random.default_rng(n + 17)
+    pos = rng.uniform(0, 700, size=(n, 3)).astype(np.float32)
+    if n > 200:
+        pos[100:200] = pos[50]  # equal keys keep input order
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32) if with_mass else None
+    keys, perm, ps, vs, ms = P.morton_sort_particles(pos, vel, mass, lib=hip)
+    k = oracle_keys(orc, pos)
+    want = oracle_perm(orc, k)
+    assert np.array_equal(perm.get(), want) and np.array_equal(keys.get(), k[want])
+    assert np.array_equal(ps.get().reshape(n, 3), pos[want])
+    assert np.array_equal(vs.get().reshape(n, 3), vel[want])
+    assert np.array_equal(ms.get(), mass[want] if with_mass else np.ones(n, np.float32))
+
+
 @pytest.mark.parametrize("n,nb", [(0, 1), (1, 1), (4, 2), (1000, 64), (10_000, 1024), (1_000_000, 32768),
                                   (200_000, 262144)])
 def test_bucket_indices_match_oracle(hip, orc, n, nb):
