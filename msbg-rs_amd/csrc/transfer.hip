@@ -192,6 +192,32 @@ __global__ void bucket_keys_kernel(const float *__restrict__ pos, int64_t n, int
     }
 }
 
+__global__ void set_flag_kernel(int32_t *p, int32_t v) { *p = v; }
+
+// bucket_keys_kernel, and *ungrouped set to 1 when some particle's block
+// precedes the previous particle's in Morton order: otherwise (Morton-sorted
+// input) each block's particles already form one run in input order, which is
+// the order the stable bucket sort gives within a block.
+__global__ void bucket_keys_grouped_kernel(const float *__restrict__ pos, int64_t n, int32_t nx, int32_t ny,
+                                           uint32_t *__restrict__ keys, uint32_t *__restrict__ idx,
+                                           int32_t *__restrict__ ungrouped) {
+    auto mblock = [](uint32_t x, uint32_t y, uint32_t z) {
+        return spread10(x >> 4) | (spread10(y >> 4) << 1) | (spread10(z >> 4) << 2);
+    };
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = trunc_index(pos[3 * i]), y = trunc_index(pos[3 * i + 1]), z = trunc_index(pos[3 * i + 2]);
+        keys[i] = (x >> 4) + (y >> 4) * (uint32_t)nx + (z >> 4) * (uint32_t)nx * (uint32_t)ny;
+        idx[i] = (uint32_t)i;
+        bool down = false;
+        if (i > 0) {
+            const uint32_t px = trunc_index(pos[3 * i - 3]), py = trunc_index(pos[3 * i - 2]),
+                           pz = trunc_index(pos[3 * i - 1]);
+            down = mblock(x, y, z) < mblock(px, py, pz);
+        }
+        if (__any(down) && (threadIdx.x & 63) == 0) *ungrouped = 1;  // one store per wave
+    }
+}
+
 // Gathers (pos, vel, mass) rows into sorted order.
 __global__ void gather_payload_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
                                       const float *__restrict__ mass, const uint32_t *__restrict__ perm, int64_t n,
@@ -1071,24 +1097,36 @@ int msbg_particles_to_grid_mac(msbg_grid *const *vel, msbg_grid *const *mass, co
     if (rc) return rc;
     *n_active = (int64_t)act.size();
     if (n == 0) return MSBG_OK;
-    // bucket by block (p2g.rs:149-158)
-    uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n, &rc);
+    // bucket by block (p2g.rs:149-158).  Particles that arrive grouped by
+    // block (Morton-sorted, as a FLIP step passes them) skip the sort and the
+    // payload gather: the runs of the input are the buckets.
+    uint32_t *keys = (uint32_t *)workspace(4, sizeof(uint32_t) * 2 * n + 2 * sizeof(int32_t), &rc);
     if (rc) return rc;
     uint32_t *perm = keys + n;
-    hipLaunchKernelGGL(bucket_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, n, (int32_t)g->nx,
-                       (int32_t)g->ny, keys, perm);
+    int32_t *d_ungrouped = (int32_t *)(perm + n);
+    hipLaunchKernelGGL(set_flag_kernel, dim3(1), dim3(1), 0, stream(), d_ungrouped, 0);
+    hipLaunchKernelGGL(bucket_keys_grouped_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, n,
+                       (int32_t)g->nx, (int32_t)g->ny, keys, perm, d_ungrouped);
     MSBG_HIP(hipGetLastError());
-    int key_bits = 1;
-    while (key_bits < 32 && ((int64_t)1 << key_bits) < g->n_blocks) ++key_bits;
-    rc = radix_sort_pairs(keys, perm, n, key_bits);
-    if (rc) return rc;
+    int32_t ungrouped = 1;
+    MSBG_HIP(hipMemcpyAsync(&ungrouped, d_ungrouped, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+    MSBG_HIP(hipStreamSynchronize(stream()));
     float *pay = (float *)workspace(7, sizeof(float) * 7 * n + sizeof(int32_t) * (2 * n + 2), &rc);
     if (rc) return rc;
-    float *spos = pay, *svel = pay + 3 * n, *smass = pay + 6 * n;
+    const float *spos = d_pos, *svel = d_vel, *smass = d_mass;
     int32_t *pblocks = (int32_t *)(pay + 7 * n), *starts = pblocks + n;
-    hipLaunchKernelGGL(gather_payload_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass, perm,
-                       n, spos, svel, smass);
-    MSBG_HIP(hipGetLastError());
+    if (ungrouped) {
+        int key_bits = 1;
+        while (key_bits < 32 && ((int64_t)1 << key_bits) < g->n_blocks) ++key_bits;
+        rc = radix_sort_pairs(keys, perm, n, key_bits);
+        if (rc) return rc;
+        hipLaunchKernelGGL(gather_payload_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream(), d_pos, d_vel, d_mass,
+                           perm, n, pay, pay + 3 * n, pay + 6 * n);
+        MSBG_HIP(hipGetLastError());
+        spos = pay;
+        svel = pay + 3 * n;
+        smass = pay + 6 * n;
+    }
     int64_t nocc = 0;
     rc = run_lengths(keys, n, pblocks, starts, &nocc);
     if (rc) return rc;

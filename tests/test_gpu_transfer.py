@@ -120,6 +120,35 @@ def test_particles_to_grid_mac_matches_oracle(pkg, orc, hip, n, s, r_p):
     print("MAC P2G n=%d: worst error / (1e-6 bound) = %.3f" % (n, worst))
 
 
+@pytest.mark.parametrize("r_p", [2.0, 3.0])
+def test_particles_to_grid_mac_grouped_input(pkg, orc, hip, r_p):
+    """Particles in Morton order (each block's particles one run, as a FLIP
+    step passes them) take the path without the bucket sort: the same 1e-6
+    bound against the float64 sums, and the same result as the shuffled input
+    up to the sums' last-bit rounding."""
+    s = (48, 40, 56)
+    rng = np.random.default_rng(31)
+    n = 20000
+    pos = rng.uniform(0.0, np.array(s) - 1e-3, size=(n, 3)).astype(np.float32)
+    pos[:300] = pos[7]  # a run of equal positions
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    mass = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    keys = np.zeros(n, np.uint32)
+    orc.orc_morton_keys(np.ascontiguousarray(pos), n, keys)
+    order = np.argsort(keys, kind="stable")
+    cap = ((s[0] + 15) // 16) * ((s[1] + 15) // 16) * ((s[2] + 15) // 16) + 8
+    sorted_mac = T.MacGrid(*s, capacity_blocks=cap, lib=hip)
+    k = T.particles_to_grid(sorted_mac, pos[order], vel[order], mass[order], r_p)
+    active, worst = check_mac_against_f64(orc, sorted_mac, s, pos[order], vel[order], mass[order], r_p)
+    assert k == active.size
+    shuffled_mac = T.MacGrid(*s, capacity_blocks=cap, lib=hip)
+    T.particles_to_grid(shuffled_mac, pos, vel, mass, r_p)
+    for a in range(3):
+        ms, mu = sorted_mac.mass[a].to_host().to_dense(), shuffled_mac.mass[a].to_host().to_dense()
+        np.testing.assert_allclose(ms, mu, rtol=1e-6, atol=0)
+    print("MAC P2G grouped input r_p=%.1f: worst error / (1e-6 bound) = %.3f" % (r_p, worst))
+
+
 def test_particles_to_grid_mac_dense_cluster(pkg, orc, hip):
     """30,000 particles in a sphere of radius 4 cells (as in a droplet): the
     block holding them is split into work items of 4,096 particles whose
