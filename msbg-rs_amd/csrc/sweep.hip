@@ -162,6 +162,97 @@ __device__ __forceinline__ void fill_tile_dense(float *tile, const GridView &g, 
     }
 }
 
+// Row-wise fill of the dense gather tile (HSX^3, x fastest) for blocks whose
+// rows lie inside the domain in x: a tile row (hy, hz) is resolved once
+// under the boundary condition `bc`; its 16 interior cells are one block row
+// (four float4 loads, or two float2 loads duplicated for a coarse neighbour)
+// and its 2H x-halo cells are resolved and loaded one by one.  A row outside
+// the domain (Dirichlet) is empty_f32.  `full == false` leaves the x-halo
+// cells of rows outside the block in y or z at 0, as fill_tile_dense does.
+template <int HALO, bool MULTI>
+__device__ __forceinline__ void fill_tile_dense_rows(float *tile, const GridView &g, const GridView &c,
+                                                     const int *s_nb, const int *s_nbc, int bx0, int by0, int bz0,
+                                                     int bc, float empty_f32, bool full) {
+    constexpr int B = 16, L = 4, HSX = B + 2 * HALO, n = 2 * HALO + 1, NTHR = kGatherThreads;
+    constexpr int NR = HSX * HSX, NQ = 4 * NR, NH = 2 * HALO * NR;
+    constexpr int QI = (NQ + NTHR - 1) / NTHR, HI = (NH + NTHR - 1) / NTHR;
+    const int x0 = bx0 * B, y0 = by0 * B, z0 = bz0 * B;
+    const int t = threadIdx.x;
+    const float *qp[QI];
+    uint32_t qcoarse = 0;
+#pragma unroll
+    for (int k = 0; k < QI; ++k) {  // interior quads: item = row * 4 + q
+        const int it = t + k * NTHR;
+        qp[k] = nullptr;
+        if (it >= NQ) continue;
+        const int r = it >> 2, q = it & 3;
+        const int hy = r % HSX, hz = r / HSX;
+        int ry, rz;
+        if (!resolve_axis(y0 + hy - HALO, g.sy - 1, bc, ry) || !resolve_axis(z0 + hz - HALO, g.sz - 1, bc, rz))
+            continue;
+        const int nb = HALO + n * ((ry >> L) - by0 + HALO) + n * n * ((rz >> L) - bz0 + HALO);
+        if (MULTI && s_nbc[nb] >= 0) {
+            qcoarse |= 1u << k;
+            qp[k] = c.slab + (int64_t)s_nbc[nb] * 512 + (((ry >> 1) & 7) << 3) + (((rz >> 1) & 7) << 6) + 2 * q;
+        } else {
+            qp[k] = g.slab + (int64_t)s_nb[nb] * 4096 + ((ry & 15) << 4) + ((rz & 15) << 8) + 4 * q;
+        }
+    }
+    const float *hp[HI];
+    uint32_t hhole = 0;
+#pragma unroll
+    for (int k = 0; k < HI; ++k) {  // x-halo cells: item = row * 2H + j; j < H left
+        const int it = t + k * NTHR;
+        hp[k] = nullptr;
+        if (it >= NH) continue;
+        const int r = it / (2 * HALO), j = it % (2 * HALO);
+        const int hy = r % HSX, hz = r / HSX;
+        const bool yh = hy < HALO || hy >= HALO + B, zh = hz < HALO || hz >= HALO + B;
+        if (!full && (yh || zh)) {
+            hhole |= 1u << k;
+            continue;
+        }
+        const int gx = j < HALO ? x0 - HALO + j : x0 + B + (j - HALO);
+        int rx, ry, rz;
+        if (resolve_axis(gx, g.sx - 1, bc, rx) && resolve_axis(y0 + hy - HALO, g.sy - 1, bc, ry) &&
+            resolve_axis(z0 + hz - HALO, g.sz - 1, bc, rz))
+            hp[k] = cell_ptr<HALO, MULTI, float>(g.slab, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
+    }
+    float4 qv[QI];
+#pragma unroll
+    for (int k = 0; k < QI; ++k) {
+        if (!qp[k]) {
+            qv[k] = make_float4(empty_f32, empty_f32, empty_f32, empty_f32);
+        } else if (MULTI && ((qcoarse >> k) & 1u)) {
+            const float2 c2 = *reinterpret_cast<const float2 *>(qp[k]);
+            qv[k] = make_float4(c2.x, c2.x, c2.y, c2.y);
+        } else {
+            qv[k] = *reinterpret_cast<const float4 *>(qp[k]);
+        }
+    }
+    float hv[HI];
+#pragma unroll
+    for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? *hp[k] : (((hhole >> k) & 1u) ? 0.0f : empty_f32);
+#pragma unroll
+    for (int k = 0; k < QI; ++k) {
+        const int it = t + k * NTHR;
+        if (it >= NQ) continue;
+        const int r = it >> 2, q = it & 3;
+        float *d = tile + r * HSX + HALO + 4 * q;
+        d[0] = qv[k].x;
+        d[1] = qv[k].y;
+        d[2] = qv[k].z;
+        d[3] = qv[k].w;
+    }
+#pragma unroll
+    for (int k = 0; k < HI; ++k) {
+        const int it = t + k * NTHR;
+        if (it >= NH) continue;
+        const int r = it / (2 * HALO), j = it % (2 * HALO);
+        tile[r * HSX + (j < HALO ? j : HALO + B + (j - HALO))] = hv[k];
+    }
+}
+
 // Distance of tile coordinate h outside the block's [H, H + 16) (0 inside).
 template <int HALO>
 __device__ __forceinline__ int out_dist(int h) {
@@ -643,17 +734,23 @@ __global__ __launch_bounds__(kGatherThreads) void gather_kernel(GridView g, Grid
                                                                 const int64_t *bids, int bc, int full,
                                                                 float empty_f32, float *tiles) {
     constexpr int HSX = 16 + 2 * HALO, NT = HSX * HSX * HSX;
-    __shared__ float tile[NT];
+    __shared__ __attribute__((aligned(16))) float tile[NT];
     __shared__ int s_nb[125];
     __shared__ int s_nbc[MULTI ? 125 : 1];
     const int64_t bid = bids[blockIdx.x];
     const int bx0 = (int)(bid % g.nx), by0 = (int)((bid / g.nx) % g.ny), bz0 = (int)(bid / ((int64_t)g.nx * g.ny));
     resolve_neighbours<HALO, MULTI>(g, c, levels, bx0, by0, bz0, bid, s_nb, s_nbc);
     __syncthreads();
-    fill_tile_dense<HALO, MULTI>(tile, g, c, s_nb, s_nbc, bx0, by0, bz0, bc, empty_f32, full != 0);
+    if (bx0 * 16 + 16 <= g.sx)
+        fill_tile_dense_rows<HALO, MULTI>(tile, g, c, s_nb, s_nbc, bx0, by0, bz0, bc, empty_f32, full != 0);
+    else
+        fill_tile_dense<HALO, MULTI>(tile, g, c, s_nb, s_nbc, bx0, by0, bz0, bc, empty_f32, full != 0);
     __syncthreads();
-    float *dst = tiles + (int64_t)blockIdx.x * NT;
-    for (int i = threadIdx.x; i < NT; i += kGatherThreads) dst[i] = tile[i];
+    // the tile is NT contiguous floats (NT and the tile stride are multiples of 4)
+    static_assert(NT % 4 == 0, "tile rows are written as float4");
+    float4 *dst = reinterpret_cast<float4 *>(tiles + (int64_t)blockIdx.x * NT);
+    const float4 *src = reinterpret_cast<const float4 *>(tile);
+    for (int i = threadIdx.x; i < NT / 4; i += kGatherThreads) dst[i] = src[i];
 }
 
 }  // namespace msbg
