@@ -526,11 +526,27 @@ __global__ __launch_bounds__(kThreads) void ir_update_pair_kernel(const PBlock *
     for (int j = 0; j < Z; ++j) {
         if (j >= Q.nz) break;
         const i64 cj = Q.c0 + 256 * j;
-        for (int i = 0; i < Q.n; ++i) {
-            float hi = x[cj + i], lo = xlo[cj + i];
-            pair_add(hi, lo, s * Sto<ST>::ld(e, cj + i));
-            x[cj + i] = hi;
-            xlo[cj + i] = lo;
+        if (Q.n == W) {  // whole row vectors: one 16-byte load of e, float4 loads and stores of hi and lo
+            Row<ST> ev;
+            ev.load(e, cj);
+#pragma unroll
+            for (int k = 0; k < W; k += 4) {
+                float4 h = *reinterpret_cast<const float4 *>(x + cj + k);
+                float4 l = *reinterpret_cast<const float4 *>(xlo + cj + k);
+                pair_add(h.x, l.x, s * ev.at(k));
+                pair_add(h.y, l.y, s * ev.at(k + 1));
+                pair_add(h.z, l.z, s * ev.at(k + 2));
+                pair_add(h.w, l.w, s * ev.at(k + 3));
+                *reinterpret_cast<float4 *>(x + cj + k) = h;
+                *reinterpret_cast<float4 *>(xlo + cj + k) = l;
+            }
+        } else {
+            for (int i = 0; i < Q.n; ++i) {
+                float hi = x[cj + i], lo = xlo[cj + i];
+                pair_add(hi, lo, s * Sto<ST>::ld(e, cj + i));
+                x[cj + i] = hi;
+                xlo[cj + i] = lo;
+            }
         }
     }
 }
