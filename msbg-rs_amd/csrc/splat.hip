@@ -1,17 +1,19 @@
 // Density min-splat: particles::splat (particles/splat.rs:92-319) for
 // D = Density (u16, channel.rs:303-375) into a u16 slab over a grid layout.
 //
-// A block's (16 + 2m)^3 window (m = ceil(r_scan)) is split into kZSlabs
-// z-slabs, one work-group each: the work-group stages its slab in LDS as u32
-// cells initialised to 65535, walks the block's particle bucket in chunks
-// (positions and footprint ranges staged in LDS, keeping the particles whose
-// footprint reaches the slab), lowers every (particle, footprint row) pair's
-// cells with an LDS atomic min of the quantised distance, then commits its
-// slab to the blocks it covers with dst = min(dst, staging).  Buckets run in
-// 8 colour passes (block-coordinate parity), so same-colour windows never
-// overlap (16 + 2m <= 32) and, the slabs of one window being disjoint, the
-// commit needs no global atomics.  min is associative and commutative, so
-// the result does not depend on any order.
+// A block's particle bucket is cut into work items of at most
+// kItemParticles particles, and each item's (16 + 2m)^3 window
+// (m = ceil(r_scan)) into kZSlabs z-slabs, one work-group each: the
+// work-group stages its slab in LDS as u32 cells initialised to 65535, walks
+// the item's particles in chunks (positions and footprint ranges staged in
+// LDS, keeping the particles whose footprint reaches the slab), lowers every
+// (particle, footprint row) pair's cells with an LDS atomic min of the
+// quantised distance, then commits its slab to the blocks it covers with
+// dst = min(dst, staging).  Buckets run in 8 colour passes (block-coordinate
+// parity), so same-colour windows never overlap (16 + 2m <= 32); the slabs of
+// one item are disjoint, and only the items of one split block (a dense
+// point-cloud instance) commit to the same cells, with an atomic min.  min is
+// associative and commutative, so the result does not depend on any order.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -33,19 +35,40 @@ constexpr int kZSlabs = 8;      // work-groups per window
 constexpr int kSlabPlanes = 4;  // ceil(28 / 8)
 constexpr int kChunk = 256;     // particles staged per pass (one per thread)
 
+// A work item of one colour pass: particles p0 .. p1 - 1 (bucketed order) of
+// occupied block blocks[k]; `shared` when the block's particles are split
+// over several items, whose windows then commit with an atomic min.
+struct DItem {
+    int32_t k, p0, p1, shared;
+};
+constexpr int kItemParticles = 1024;  // particles per item: a dense block (one point-cloud instance) is split
+
+// dst = min(dst, v) on a 16-bit cell, atomically (compare-and-swap on its 32-bit word).
+__device__ __forceinline__ void atomic_min_u16(uint16_t *p, uint32_t v) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<uintptr_t>(p) & ~(uintptr_t)3);
+    const int sh = (reinterpret_cast<uintptr_t>(p) & 2) ? 16 : 0;
+    uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (((old >> sh) & 0xffffu) > v) {
+        const uint32_t want = (old & ~(0xffffu << sh)) | (v << sh);
+        const uint32_t seen = atomicCAS(w, old, want);
+        if (seen == old) break;
+        old = seen;
+    }
+}
+
 __global__ __launch_bounds__(256) void density_splat_kernel(SplatParams P, const int32_t *__restrict__ b2s,
                                                             uint16_t *slab, const float *__restrict__ pos,
                                                             const int32_t *__restrict__ blocks,
-                                                            const int32_t *__restrict__ starts, int color) {
+                                                            const DItem *__restrict__ items) {
     __shared__ uint32_t st[28 * 28 * kSlabPlanes];
     __shared__ int rslot[27];
     __shared__ float s_p[kChunk][3];      // particle position - 0.5
     __shared__ int16_t s_r[kChunk][6];    // footprint: ix1, ix2, iy1, iy2, iz1, iz2 (window coordinates, slab-clipped in z)
     __shared__ int s_n;
-    const int k = blockIdx.x / kZSlabs, zs = blockIdx.x % kZSlabs;
-    const int bid = blocks[k];
+    const DItem item = items[blockIdx.x / kZSlabs];
+    const int zs = blockIdx.x % kZSlabs;
+    const int bid = blocks[item.k];
     const int bx = bid % P.nx, by = (bid / P.nx) % P.ny, bz = bid / (P.nx * P.ny);
-    if (((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) != color) return;  // uniform per work-group
     const int W = P.W, m = P.m;
     const int z0 = zs * W / kZSlabs, z1 = (zs + 1) * W / kZSlabs;  // window planes [z0, z1) of this work-group
     const int cells = W * W * (z1 - z0);
@@ -68,7 +91,7 @@ __global__ __launch_bounds__(256) void density_splat_kernel(SplatParams P, const
     }
     // stage (splat.rs:179-225): footprint ceil(p - r - 0.5) .. floor(p + r - 0.5),
     // clipped to the window; d2 = dx*dx + (dz*dz + dy*dy)
-    const int p0 = starts[k], p1 = starts[k + 1];
+    const int p0 = item.p0, p1 = item.p1;
     const int side = 2 * m + 1;
     for (int c0 = p0; c0 < p1; c0 += kChunk) {
         if (threadIdx.x == 0) s_n = 0;
@@ -122,7 +145,10 @@ __global__ __launch_bounds__(256) void density_splat_kernel(SplatParams P, const
         if (slot < 0) continue;
         const int gx = b0x + cx - m, gy = b0y + cy - m, gz = b0z + cz - m;
         uint16_t *d = slab + ((int64_t)slot << 12) + ((gx & 15) | ((gy & 15) << 4) | ((gz & 15) << 8));
-        if (v < *d) *d = (uint16_t)v;
+        if (item.shared)
+            atomic_min_u16(d, v);
+        else if (v < *d)
+            *d = (uint16_t)v;
     }
 }
 
@@ -208,10 +234,38 @@ int msbg_density_splat(msbg_grid *g, uint16_t *d_slab16, int64_t slab_slots, con
     P.r = r_scan;
     P.dsq_max = r_scan * r_scan;
     P.inv = 1.0f / P.dsq_max;
-    MSBG_CHECK(nb * kZSlabs < ((int64_t)1 << 31), MSBG_EINVAL, "density splat: too many occupied blocks");
-    for (int color = 0; color < 8 && nb; ++color)
-        density_splat_kernel<<<(int)(nb * kZSlabs), 256, 0, s>>>(P, g->d_b2s, d_slab16, d_sorted, d_blocks, d_starts,
-                                                                color);
+    // colour lists of work items (the 8 parity buckets of splat.rs:114-123),
+    // each block's particles cut into items of at most kItemParticles
+    std::vector<int32_t> hb((size_t)nb), hs((size_t)nb + 1);
+    MSBG_HIP(hipMemcpyAsync(hb.data(), d_blocks, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+    MSBG_HIP(hipMemcpyAsync(hs.data(), d_starts, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, s));
+    MSBG_HIP(hipStreamSynchronize(s));
+    std::vector<DItem> order;
+    int64_t cnt[8] = {0};
+    for (int c = 0; c < 8; ++c)
+        for (int64_t i = 0; i < nb; ++i) {
+            const int64_t b = hb[i];
+            const int64_t bx = b % g->nx, by = (b / g->nx) % g->ny, bz = b / (g->nx * g->ny);
+            if ((int)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)) != c) continue;
+            const int32_t p0 = hs[i], p1 = hs[i + 1];
+            const int32_t shared = p1 - p0 > kItemParticles ? 1 : 0;
+            for (int32_t q = p0; q < p1; q += kItemParticles) {
+                order.push_back(DItem{(int32_t)i, q, std::min(p1, q + kItemParticles), shared});
+                ++cnt[c];
+            }
+        }
+    MSBG_CHECK((int64_t)order.size() * kZSlabs < ((int64_t)1 << 31), MSBG_EINVAL,
+               "density splat: too many work items");
+    DItem *d_items = (DItem *)workspace(23, sizeof(DItem) * std::max<size_t>(order.size(), 1), &rc);
+    if (rc) return rc;
+    MSBG_HIP(hipMemcpyAsync(d_items, order.data(), sizeof(DItem) * order.size(), hipMemcpyHostToDevice, s));
+    int64_t off = 0;
+    for (int c = 0; c < 8; ++c) {
+        if (cnt[c])
+            density_splat_kernel<<<(int)(cnt[c] * kZSlabs), 256, 0, s>>>(P, g->d_b2s, d_slab16, d_sorted, d_blocks,
+                                                                       d_items + off);
+        off += cnt[c];
+    }
     MSBG_HIP(hipGetLastError());
     MSBG_HIP(hipStreamSynchronize(s));
     return MSBG_OK;

@@ -317,6 +317,38 @@ def test_density_splat_bitexact(pkg, orc, hip, r_scan, s, n):
         assert got[b2s[0], 8 | (8 << 4) | (8 << 8)] == 3072
 
 
+def test_density_splat_dense_block_bitexact(pkg, orc, hip):
+    """A point-cloud instance packed into one block (7,000 particles within 3
+    cells of a point, as the surface benchmark places its instances): the
+    block's bucket is split into several work items whose windows commit with
+    an atomic 16-bit min, and the result is still the oracle's bit for bit."""
+    from msbg_rs_amd.particles import DensityGrid
+    from test_oracle_splat import density_grid, run
+    import ctypes as C
+    fn = orc.lib.orc_density_splat
+    fn.restype = None
+    fn.argtypes = [C.c_int64] * 3 + [np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS"), C.c_int64, C.c_float,
+                                     C.c_int]
+    s, r_scan = (48, 48, 48), 4.0
+    rng = np.random.default_rng(77)
+    d = rng.normal(size=(7000, 3))
+    d *= (3.0 * rng.uniform(0, 1, (7000, 1)) ** (1 / 3)) / np.linalg.norm(d, axis=1, keepdims=True)
+    cluster = np.array([23.7, 24.2, 22.9]) + d
+    sparse = rng.uniform(0.0, np.array(s) - 1e-3, size=(1500, 3))
+    pos = np.concatenate([cluster, sparse]).astype(np.float32)
+    rng.shuffle(pos)
+    g, slab = density_grid(pkg, orc, s, pos, r_scan)
+    want = run(fn, g, slab, pos, r_scan)
+    dg = DensityGrid(*s, capacity_blocks=g.n_slots + 4, lib=hip)
+    k = dg.splat(pos, r_scan)
+    b2s, got = dg.to_host()
+    assert k == g.n_slots - 2
+    act = np.nonzero(g.b2s >= 2)[0]
+    assert np.array_equal(got[b2s[act]], want[g.b2s[act]])
+
+
 @pytest.mark.parametrize("s,frac", [((32, 32, 32), 1.0), ((40, 33, 48), 0.7), ((64, 48, 80), 0.5)])
 def test_mac_divergence_bitexact(pkg, orc, hip, s, frac):
     from test_oracle_transfer import mac_channels, oracle_divergence
