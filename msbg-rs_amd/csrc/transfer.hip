@@ -719,7 +719,9 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
                                                           const float4 *__restrict__ pm,
                                                           const float4 *__restrict__ pu) {
     __shared__ int s_nb[27];
-    __shared__ uint2 s_rng[9][256];  // per-lane voxel ranges of the current plane (each lane its own column)
+    // voxel ranges of the block's 18 x 18 plane window (the block's columns
+    // and a 1-voxel ring), double-buffered: plane v + 1 loads during plane v
+    __shared__ uint2 s_pl[2][18 * 18];
     const int64_t bid = active[blockIdx.x];
     const int bx = (int)(bid % mg.nx), by = (int)((bid / mg.nx) % mg.ny), bz = (int)(bid / ((int64_t)mg.nx * mg.ny));
     if (threadIdx.x < 27) {
@@ -744,33 +746,49 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
         v1.slab[(int64_t)s1 * 4096 + vid] = pos_m ? a.u1 / a.m : 0.0f;
         v2.slab[(int64_t)s2 * 4096 + vid] = pos_m ? a.u2 / a.m : 0.0f;
     };
-    for (int lv = -1; lv <= 16; ++lv) {
-        const int vz = cz0 + lv;
-        if (vz >= 0 && vz < mg.sz) {
-            // The plane's 9 voxel ranges of this lane, (dy, dx) in order; an
-            // outside or absent voxel is an empty range.
+    // Ranges of window voxels idx and idx + 256 (idx < 324) of plane vz; an
+    // outside or absent voxel is an empty range.
+    auto plane_ranges = [&](int vz, uint2 *r) {
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int vy = cy + t / 3 - 1, vx = cx + t % 3 - 1;
-                uint2 r = make_uint2(0u, 0u);
-                if (vy >= 0 && vy < mg.sy && vx >= 0 && vx < mg.sx) {
-                    const int slot = s_nb[((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1)];
-                    if (slot >= 2) {
-                        const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
-                        r = make_uint2(vbeg[c], vend[c]);
-                    }
-                }
-                s_rng[t][threadIdx.x] = r;
-            }
-            // One pass over the concatenation of the 9 ranges: a wave steps
-            // as often as its fullest lane has particles, not once per voxel
-            // per the fullest lane of that voxel.  Per cell, the particles
-            // still arrive voxel by voxel in (dy, dx) order, sorted within.
+        for (int h = 0; h < 2; ++h) {
+            const int idx = threadIdx.x + 256 * h;
+            r[h] = make_uint2(0u, 0u);
+            if (idx >= 18 * 18 || vz < 0 || vz >= mg.sz) continue;
+            const int vy = by * 16 - 1 + idx / 18, vx = bx * 16 - 1 + idx % 18;
+            if (vy < 0 || vy >= mg.sy || vx < 0 || vx >= mg.sx) continue;
+            const int slot = s_nb[((vx >> 4) - bx + 1) + 3 * ((vy >> 4) - by + 1) + 9 * ((vz >> 4) - bz + 1)];
+            if (slot < 2) continue;
+            const int64_t c = (int64_t)slot * 4096 + ((vx & 15) | ((vy & 15) << 4) | ((vz & 15) << 8));
+            r[h] = make_uint2(vbeg[c], vend[c]);
+        }
+    };
+    auto put_plane = [&](int buf, const uint2 *r) {
+        s_pl[buf][threadIdx.x] = r[0];
+        if (threadIdx.x + 256 < 18 * 18) s_pl[buf][threadIdx.x + 256] = r[1];
+    };
+    uint2 nxt[2];
+    plane_ranges(cz0 - 1, nxt);
+    put_plane(0, nxt);
+    __syncthreads();
+    for (int lv = -1; lv <= 16; ++lv) {
+        const int vz = cz0 + lv, cur = (lv + 1) & 1;
+        if (lv < 16) plane_ranges(vz + 1, nxt);  // in flight while this plane's particles are added
+        if (vz >= 0 && vz < mg.sz) {
+            // This lane's 9 voxel ranges, (dy, dx) in order, are window
+            // voxels (ly + dy + 1, lx + dx + 1).  One pass over their
+            // concatenation: a wave steps as often as its fullest lane has
+            // particles, not once per voxel per the fullest lane of that
+            // voxel.  Per cell, the particles still arrive voxel by voxel in
+            // (dy, dx) order, sorted within.
+            const uint2 *pl = s_pl[cur] + ly * 18 + lx;
+            auto rng = [&](int t) { return pl[(t / 3) * 18 + t % 3]; };
             int t = 0;
-            uint32_t j = s_rng[0][threadIdx.x].x, e = s_rng[0][threadIdx.x].y;
+            uint2 rt = rng(0);
+            uint32_t j = rt.x, e = rt.y;
             while (j >= e && ++t < 9) {
-                j = s_rng[t][threadIdx.x].x;
-                e = s_rng[t][threadIdx.x].y;
+                rt = rng(t);
+                j = rt.x;
+                e = rt.y;
             }
             float4 pn = make_float4(0.f, 0.f, 0.f, 0.f), un = pn;
             if (t < 9) {
@@ -781,8 +799,9 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
                 const float4 p = pn, u = un;
                 ++j;
                 while (j >= e && ++t < 9) {
-                    j = s_rng[t][threadIdx.x].x;
-                    e = s_rng[t][threadIdx.x].y;
+                    rt = rng(t);
+                    j = rt.x;
+                    e = rt.y;
                 }
                 if (t < 9) {  // the next particle's records are in flight while this one is added
                     pn = pm[j];
@@ -794,6 +813,8 @@ __global__ __launch_bounds__(256) void p2g_bspline_kernel(GridView mg, GridView 
                 if (lv <= 14) acc_add<ORDER>(hi, wxy, p.z, cz0 + lv + 1, p.w, u);
             }
         }
+        if (lv < 16) put_plane(cur ^ 1, nxt);  // buffer cur ^ 1 was last read before the previous barrier
+        __syncthreads();
         if (lv >= 1) store(lv - 1, lo);  // cell lv-1 has its planes lv-2, lv-1, lv
         lo = mid;
         mid = hi;
