@@ -370,18 +370,33 @@ def test_mac_divergence_bitexact(pkg, orc, hip, s, frac):
         assert_bitexact(got.slab[got.b2s[b]], want.slab[want.b2s[b]], "div block %d" % b)
 
 
-@pytest.mark.parametrize("interp,bc", [(0, 0), (1, 0), (0, 1)])
-def test_g2p_flip_matches_sample_twice_and_blend(pkg, orc, hip, interp, bc):
-    """msbg_g2p_flip: bit for bit the oracle's two MAC samples and blend."""
+def morton_order(pos):
+    """Permutation sorting positions by the Morton code of their cell (cells
+    clipped to [0, 1023]): consecutive particles are spatial neighbours."""
+    c = np.clip(np.floor(pos), 0, 1023).astype(np.uint64)
+    key = np.zeros(pos.shape[0], np.uint64)
+    for bit in range(10):
+        for d in range(3):
+            key |= ((c[:, d] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + d)
+    return np.argsort(key, kind="stable")
+
+
+@pytest.mark.parametrize("interp,bc,n,sort", [(0, 0, 20000, False), (1, 0, 20000, False), (0, 1, 20000, False),
+                                              (0, 0, 300000, True), (0, 2, 300000, True), (0, 1, 50000, True)])
+def test_g2p_flip_matches_sample_twice_and_blend(pkg, orc, hip, interp, bc, n, sort):
+    """msbg_g2p_flip: bit for bit the oracle's two MAC samples and blend, on
+    random and on Morton-sorted particles (the order a FLIP step passes),
+    some of them near or beyond the domain's faces."""
     s = (40, 33, 48)
     f = lambda a, X, Y, Z: np.sin(0.21 * X + a) * np.cos(0.13 * Y) + 0.03 * Z
     g = lambda a, X, Y, Z: np.cos(0.17 * X) * np.sin(0.09 * Y + a) - 0.02 * Z
     from test_gpu_flip import mac_from_channels
     cu, cut = mac_channels(pkg, s, f), mac_channels(pkg, s, g)  # every block: one slot table for all channels
     mu, mut = mac_from_channels(pkg, hip, cu), mac_from_channels(pkg, hip, cut)
-    rng = np.random.default_rng(9)
-    n = 20000
+    rng = np.random.default_rng(9 + n)
     pos = rng.uniform(-1.0, np.array(s) + 1.0, size=(n, 3)).astype(np.float32)  # some outside the domain
+    if sort:
+        pos = pos[morton_order(pos)]
     uo = rng.normal(size=(n, 3)).astype(np.float32)
     for alpha in (0.0, 0.95, 1.0):
         got = T.g2p_flip(mu, mut, pos, uo, alpha, interp, bc).get()
