@@ -469,6 +469,15 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
                 asm volatile("" : "+v"(cx[o]), "+v"(cx2[o]));
             }
             const int base = ((iz1 - z0s) * MSX + (iy1 - y0s)) * MSX + ix1 - x0s;
+            // A cell can hold a contribution 0 < |q| < 2^24 only if its weight
+            // is below wthr: |q| = fl(w |sq|) < 2^24 needs w |sq| < 2^24, and
+            // wthr >= 2^24 / |sq| on every channel with sq != 0 (the quotient
+            // rounded, then raised by 2^-21).  Cells with 0 < w < wthr take the
+            // vote, a superset of those that need it.
+            float wthr = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                if (sq[k] != 0.0f) wthr = fmaxf(wthr, (kExact / fabsf(sq[k])) * (1.0f + 0x1p-21f));
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz)
 #pragma unroll
@@ -486,19 +495,12 @@ __global__ __launch_bounds__(kSplatThreads) void mac_splat_kernel(GridView gm, G
                         // contribution is 0 or at least 2^24 in magnitude,
                         // so its magnitude converts exactly and the signed
                         // add (zero included) goes straight to the window.
-                        float q[NC];
-                        bool odd = false;
-#pragma unroll
-                        for (int k = 0; k < NC; ++k) {
-                            q[k] = w * sq[k];
-                            odd = odd | ((fabsf(q[k]) < kExact) & (q[k] != 0.0f));  // no short circuit
-                        }
-                        if (__any(odd)) {
+                        if (__any((w > 0.0f) & (w < wthr))) {  // no short circuit
                             contrib(row + ox, w, sf, sq);
                         } else {
 #pragma unroll
                             for (int k = 0; k < NC; ++k)
-                                atomicAdd(&acc[k * NS + row + ox], (integral_mag_u64(q[k]) ^ sgn[k]) - sgn[k]);
+                                atomicAdd(&acc[k * NS + row + ox], (integral_mag_u64(w * sq[k]) ^ sgn[k]) - sgn[k]);
                         }
                     }
                 }
