@@ -67,7 +67,10 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 // them through HBM and measured 30% slower per tile.
 constexpr int64_t kForwardChunk = 512;
 constexpr int kZS = 4;   // output z-planes per work-group
-constexpr int kCG = 4;   // 16-cell groups per wave pass
+// Two 16-cell groups per wave pass: with two MFMA chains the compiler keeps
+// the next tap pair's B operands in flight (106 VGPRs, 4 waves per SIMD); with
+// four it runs each LDS read and its MFMA back to back to stay within 128.
+constexpr int kCG = 2;   // 16-cell groups per wave pass
 constexpr int kConvThreads = 512;  // 8 waves share one work-group's staged planes
 constexpr int kTP = 14;  // tap pairs of a 16-channel layer
 constexpr int kWB = kTP * 64;  // A-operand vectors (16 B) per layer
