@@ -446,9 +446,12 @@ __device__ __forceinline__ void sweep_block16(GridView g, GridView c, TS *__rest
         else
             store_back(out + z * 256, v);
     };
-    // T(dx, dy, z): tile value at block cell (x+dx, y+dy, z).
+    // T(dx, dy, z): tile value at block cell (x+dx, y+dy, z).  The lane's
+    // base is the low corner (x - H, y - H, zs - H) of its stencils, so every
+    // read is the base plus a constant (z is zs plus a constant at each call).
+    const float *tb = tile + tile_at<HALO>(x, y, zs);
     auto T = [&](int dx, int dy, int z) -> float {
-        return tile[tile_at<HALO>(x + HALO + dx, y + HALO + dy, z + HALO)];
+        return tb[((z - zs + HALO) * HSX + (dy + HALO)) * kPitch + (dx + HALO)];
     };
     auto cross = [&](int z) -> Cross { return Cross{T(0, 0, z), T(-1, 0, z), T(1, 0, z), T(0, -1, z), T(0, 1, z)}; };
 
