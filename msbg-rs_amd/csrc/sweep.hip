@@ -38,9 +38,21 @@ constexpr int kPitch = 24;
 // math/stencil.rs:162-174), saturated to the u16 range.
 __device__ __forceinline__ float deq(float v) { return v; }
 __device__ __forceinline__ float deq(uint16_t v) { return (float)v * (1.0f / 65535.0f); }
-__device__ __forceinline__ float4 load4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+// Slab reads through the global address space: the slabs are device
+// allocations, and a global load (unlike a flat one) leaves the LDS counter
+// alone, so the wait for one load does not also wait for LDS traffic.
+template <typename T>
+__device__ __forceinline__ T ld_global(const void *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+typedef float gf4 __attribute__((ext_vector_type(4)));  // native vectors: HIP's float4 copies through a generic reference
+typedef uint32_t gu2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 load4(const float *p) {
+    const gf4 v = ld_global<gf4>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ float4 load4(const uint16_t *p) {
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    const gu2 u = ld_global<gu2>(p);
     return make_float4(deq((uint16_t)(u.x & 0xffffu)), deq((uint16_t)(u.x >> 16)), deq((uint16_t)(u.y & 0xffffu)),
                        deq((uint16_t)(u.y >> 16)));
 }
@@ -48,7 +60,7 @@ __device__ __forceinline__ float4 load4(const uint16_t *p) {
 // stochastic-rounding stream and is done in sweep_block16.
 __device__ __forceinline__ float deq(uint8_t v) { return (float)v * (1.0f / 255.0f); }
 __device__ __forceinline__ float4 load4(const uint8_t *p) {
-    const uint32_t u = *reinterpret_cast<const uint32_t *>(p);
+    const uint32_t u = ld_global<uint32_t>(p);
     return make_float4(deq((uint8_t)(u & 0xffu)), deq((uint8_t)((u >> 8) & 0xffu)), deq((uint8_t)((u >> 16) & 0xffu)),
                        deq((uint8_t)(u >> 24)));
 }
@@ -326,9 +338,11 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const 
             continue;
         const int nb = HALO + n * ((ry >> L) - by0 + HALO) + n * n * ((rz >> L) - bz0 + HALO);
         if (MULTI && s_nbc[nb] >= 0) {
+            // the aligned half of the 8-cell coarse row that holds this
+            // quad's two coarse cells (its elements 2 (q & 1), + 1)
             qcoarse |= 1u << k;
             qp[k] = reinterpret_cast<const TS *>(c.slab + (int64_t)s_nbc[nb] * 512 + (((ry >> 1) & 7) << 3) +
-                                                 (((rz >> 1) & 7) << 6) + 2 * q);
+                                                 (((rz >> 1) & 7) << 6) + 4 * (q >> 1));
         } else {
             qp[k] = fs + (int64_t)s_nb[nb] * 4096 + ((ry & 15) << 4) + ((rz & 15) << 8) + 4 * q;
         }
@@ -358,20 +372,34 @@ __device__ __forceinline__ void fill_tile_rows(float *tile, const TS *fs, const 
             hp[k] = cell_ptr<HALO, MULTI, TS>(fs, c, s_nb, s_nbc, bx0, by0, bz0, rx, ry, rz);
     }
     float4 qv[QI];
-#pragma unroll
-    for (int k = 0; k < QI; ++k) {
-        if (!qp[k]) {
-            qv[k] = make_float4(empty_f32, empty_f32, empty_f32, empty_f32);
-        } else if (MULTI && ((qcoarse >> k) & 1u)) {
-            const float2 c2 = *reinterpret_cast<const float2 *>(qp[k]);
-            qv[k] = make_float4(c2.x, c2.x, c2.y, c2.y);
-        } else {
-            qv[k] = load4(qp[k]);
-        }
-    }
     float hv[HI];
+    if constexpr (MULTI) {
+        // One aligned 16-byte load per item, fine or coarse, none behind a
+        // branch, so a lane's loads are all in flight together; an item with
+        // no source reads the slab's first cells and takes the empty value.
+        float4 raw[QI];
 #pragma unroll
-    for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? deq(*hp[k]) : empty_f32;
+        for (int k = 0; k < QI; ++k) raw[k] = load4(qp[k] ? qp[k] : fs);
+#pragma unroll
+        for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? deq(ld_global<TS>(hp[k])) : empty_f32;
+#pragma unroll
+        for (int k = 0; k < QI; ++k) {
+            const int q = (t + k * NTHR) & 3;
+            if (!qp[k])
+                qv[k] = make_float4(empty_f32, empty_f32, empty_f32, empty_f32);
+            else if ((qcoarse >> k) & 1u)
+                qv[k] = (q & 1) ? make_float4(raw[k].z, raw[k].z, raw[k].w, raw[k].w)
+                                : make_float4(raw[k].x, raw[k].x, raw[k].y, raw[k].y);
+            else
+                qv[k] = raw[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < QI; ++k)
+            qv[k] = qp[k] ? load4(qp[k]) : make_float4(empty_f32, empty_f32, empty_f32, empty_f32);
+#pragma unroll
+        for (int k = 0; k < HI; ++k) hv[k] = hp[k] ? deq(ld_global<TS>(hp[k])) : empty_f32;
+    }
 #pragma unroll
     for (int k = 0; k < QI; ++k)
         if ((qwrite >> k) & 1u) {
