@@ -49,6 +49,33 @@ __device__ void gather_map_dev(const GridView &g, int ix0, int iy0, int iz0, int
     // in bounds where it does not (a Neumann reflection across a partial last
     // block), where the reference's slot bounds check panics (gather.rs:143-146).
     const float empty = g.slab[0];
+    if (dmax == 1) {
+        // The 8 cells' slots and values are each loaded in one batch (a
+        // window across a block face reads up to 8 blocks): every lookup is
+        // issued before any is used.  Same cells, same values as below.
+        int64_t bid[8];
+        int vid[8];
+        uint32_t ok = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            int rx, ry, rz;
+            const bool in = resolve_axis(ix0 + (c & 1), g.sx - 1, bc, rx) &&
+                            resolve_axis(iy0 + ((c >> 1) & 1), g.sy - 1, bc, ry) &&
+                            resolve_axis(iz0 + (c >> 2), g.sz - 1, bc, rz);
+            bid[c] = in ? (rx >> 4) + (int64_t)(ry >> 4) * g.nx + (int64_t)(rz >> 4) * g.nx * g.ny : 0;
+            vid[c] = in ? (rx & 15) | ((ry & 15) << 4) | ((rz & 15) << 8) : 0;
+            ok |= (uint32_t)in << c;
+        }
+        int slot[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) slot[c] = g.b2s[bid[c]];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) out[c] = g.slab[(int64_t)slot[c] * 4096 + vid[c]];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (!((ok >> c) & 1u)) out[c] = empty;
+        return;
+    }
     int s = 0;
     for (int k = 0; k < span; ++k) {
         int rz;
