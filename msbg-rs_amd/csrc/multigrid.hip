@@ -96,12 +96,16 @@ __device__ __forceinline__ float blk_residual(const float *__restrict__ x, const
                                               int cx, int cy, int cz, float w, float *diag_out) {
     constexpr int B3 = B * B * B, h = 16 / B;
     const i64 i = k * B3 + cx + B * (cy + B * cz);
-    const int c[3] = {cx, cy, cz}, org[3] = {Bk.ox, Bk.oy, Bk.oz}, dom[3] = {d.sx, d.sy, d.sz};
-    float s = 0.0f, dg = 0.0f;
+    const int c[3] = {cx, cy, cz};
+    // Every face's neighbour cell first, then the neighbour blocks of the
+    // faces that leave this block in one batch of loads (they depend on the
+    // cell only), then all six values in one batch: two dependent round trips
+    // instead of one or two per face.  A load whose value goes unused reads a
+    // valid address (the block's centre entry, the block's own cells).
+    int qi[6], tix[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         const int a = f >> 1, dir = (f & 1) ? 1 : -1;
-        if (dir > 0 && org[a] + (c[a] + 1) * h >= dom[a]) continue;  // beyond the domain: Neumann
         int q[3] = {cx, cy, cz}, o[3] = {0, 0, 0};
         q[a] += dir;
         if (q[a] < 0) {
@@ -111,16 +115,34 @@ __device__ __forceinline__ float blk_residual(const float *__restrict__ x, const
             q[a] -= B;
             o[a] = 1;
         }
-        const i64 nbk = (o[0] | o[1] | o[2]) ? (i64)T[nb27_at(o[0], o[1], o[2])] : k;
-        if (nbk >= 0) {
-            s += x[nbk * B3 + q[0] + B * (q[1] + B * q[2])];
+        qi[f] = q[0] + B * (q[1] + B * q[2]);
+        tix[f] = nb27_at(o[0], o[1], o[2]);  // 13 (the block itself) when the face stays inside
+    }
+    int32_t tv[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) tv[f] = T[tix[f]];
+    i64 nbk[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nbk[f] = tix[f] == 13 ? k : (i64)tv[f];
+    float xv[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) xv[f] = x[(nbk[f] >= 0 ? nbk[f] : k) * B3 + qi[f]];
+    const float bi = b[i], xc = x[i];
+    const int org[3] = {Bk.ox, Bk.oy, Bk.oz}, dom[3] = {d.sx, d.sy, d.sz};
+    float s = 0.0f, dg = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int a = f >> 1, dir = (f & 1) ? 1 : -1;
+        if (dir > 0 && org[a] + (c[a] + 1) * h >= dom[a]) continue;  // beyond the domain: Neumann
+        if (nbk[f] >= 0) {
+            s += xv[f];
             dg += 1.0f;
-        } else if (nbk == -1) {
+        } else if (nbk[f] == -1) {
             dg += w;
         }
     }
     *diag_out = dg;
-    return b[i] + s - dg * x[i];
+    return bi + s - dg * xc;
 }
 
 // One half sweep over the cells of one colour: thread j of listed block k
