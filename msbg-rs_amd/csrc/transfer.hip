@@ -1356,20 +1356,33 @@ __device__ __forceinline__ float voxel_at(const GridView &g, int x, int y, int z
     return g.slab[((int64_t)g.b2s[bid] << 12) + ((x & 15) | ((y & 15) << 4) | ((z & 15) << 8))];
 }
 
+// The divergence and projection kernels run 16 work-groups of 256 cells per
+// listed block, so the block's slot lookups (its own and its neighbours')
+// are uniform over a work-group and go out once, together; each lane then
+// issues its value loads together.  The values are those voxel_at reads.
 __global__ __launch_bounds__(256) void divergence_kernel(GridView u, GridView v, GridView w, GridView d,
                                                          const int32_t *__restrict__ bids, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n * 4096) return;
-    const int64_t bid = bids[i >> 12];
-    const int vid = (int)(i & 4095);
+    const int64_t k = (int64_t)(blockIdx.x >> 4);
+    if (k >= n) return;
+    const int vid = (int)((blockIdx.x & 15) << 8) | (int)threadIdx.x;
+    const int64_t bid = bids[k];
     const int slot = d.b2s[bid];
     if (slot < 2) return;
-    const int x = (int)(bid % d.nx) * 16 + (vid & 15), y = (int)((bid / d.nx) % d.ny) * 16 + ((vid >> 4) & 15),
-              z = (int)(bid / ((int64_t)d.nx * d.ny)) * 16 + (vid >> 8);
+    const int64_t nxy = (int64_t)d.nx * d.ny;
+    const int bx = (int)(bid % d.nx), by = (int)((bid / d.nx) % d.ny), bz = (int)(bid / nxy);
+    const int lx = vid & 15, ly = (vid >> 4) & 15, lz = vid >> 8;
+    const int x = bx * 16 + lx, y = by * 16 + ly, z = bz * 16 + lz;
     if (x >= d.sx || y >= d.sy || z >= d.sz) return;
-    const float ux1 = voxel_at(u, x, y, z), ux0 = x == 0 ? 0.0f : voxel_at(u, x - 1, y, z);
-    const float vy1 = voxel_at(v, x, y, z), vy0 = y == 0 ? 0.0f : voxel_at(v, x, y - 1, z);
-    const float wz1 = voxel_at(w, x, y, z), wz0 = z == 0 ? 0.0f : voxel_at(w, x, y, z - 1);
+    // this block's slots in the face grids and its -x / -y / -z neighbours' (itself where none exists)
+    const int64_t su = (int64_t)u.b2s[bid] << 12, sv = (int64_t)v.b2s[bid] << 12, sw = (int64_t)w.b2s[bid] << 12;
+    const int64_t sux = (int64_t)u.b2s[bx > 0 ? bid - 1 : bid] << 12;
+    const int64_t svy = (int64_t)v.b2s[by > 0 ? bid - d.nx : bid] << 12;
+    const int64_t swz = (int64_t)w.b2s[bz > 0 ? bid - nxy : bid] << 12;
+    const float ux1 = u.slab[su + vid], vy1 = v.slab[sv + vid], wz1 = w.slab[sw + vid];
+    const float um = u.slab[lx > 0 ? su + vid - 1 : sux + vid + 15];
+    const float vm = v.slab[ly > 0 ? sv + vid - 16 : svy + vid + 240];
+    const float wm = w.slab[lz > 0 ? sw + vid - 256 : swz + vid + 3840];
+    const float ux0 = x == 0 ? 0.0f : um, vy0 = y == 0 ? 0.0f : vm, wz0 = z == 0 ? 0.0f : wm;
     d.slab[((int64_t)slot << 12) + vid] = ((ux1 - ux0) + (vy1 - vy0)) + (wz1 - wz0);
 }
 
@@ -1383,31 +1396,42 @@ __global__ __launch_bounds__(256) void subtract_gradient_kernel(GridView u, Grid
                                                                 const float *__restrict__ plo,
                                                                 const int32_t *__restrict__ bids, int64_t n,
                                                                 float scale) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n * 4096) return;
-    const int64_t bid = bids[i >> 12];
-    const int vid = (int)(i & 4095);
-    const int x = (int)(bid % p.nx) * 16 + (vid & 15), y = (int)((bid / p.nx) % p.ny) * 16 + ((vid >> 4) & 15),
-              z = (int)(bid / ((int64_t)p.nx * p.ny)) * 16 + (vid >> 8);
+    const int64_t k = (int64_t)(blockIdx.x >> 4);
+    if (k >= n) return;
+    const int vid = (int)((blockIdx.x & 15) << 8) | (int)threadIdx.x;
+    const int64_t bid = bids[k];
+    const int64_t nxy = (int64_t)p.nx * p.ny;
+    const int bx = (int)(bid % p.nx), by = (int)((bid / p.nx) % p.ny), bz = (int)(bid / nxy);
+    const int lx = vid & 15, ly = (vid >> 4) & 15, lz = vid >> 8;
+    const int x = bx * 16 + lx, y = by * 16 + ly, z = bz * 16 + lz;
     if (x >= p.sx || y >= p.sy || z >= p.sz) return;
-    const float pc = voxel_at(p, x, y, z);
-    const float pn[3] = {x + 1 < p.sx ? voxel_at(p, x + 1, y, z) : pc, y + 1 < p.sy ? voxel_at(p, x, y + 1, z) : pc,
-                         z + 1 < p.sz ? voxel_at(p, x, y, z + 1) : pc};
+    // p's slots of this block and of its +x / +y / +z neighbours (itself where none exists)
+    const int64_t sp = (int64_t)p.b2s[bid] << 12;
+    const int64_t spx = (int64_t)p.b2s[bx + 1 < p.nx ? bid + 1 : bid] << 12;
+    const int64_t spy = (int64_t)p.b2s[by + 1 < p.ny ? bid + p.nx : bid] << 12;
+    const int64_t spz = (int64_t)p.b2s[bz + 1 < p.nz ? bid + nxy : bid] << 12;
+    const int64_t cx = lx < 15 ? sp + vid + 1 : spx + vid - 15, cy = ly < 15 ? sp + vid + 16 : spy + vid - 240,
+                  cz = lz < 15 ? sp + vid + 256 : spz + vid - 3840;
+    const bool hx = x + 1 < p.sx, hy = y + 1 < p.sy, hz = z + 1 < p.sz;  // the neighbour is in the domain
+    const float pc = p.slab[sp + vid];
+    const float px = p.slab[hx ? cx : sp + vid], py = p.slab[hy ? cy : sp + vid], pz = p.slab[hz ? cz : sp + vid];
+    const float pn[3] = {hx ? px : pc, hy ? py : pc, hz ? pz : pc};
     float lc = 0.0f, ln[3] = {0.0f, 0.0f, 0.0f};
-    if constexpr (LO) {
-        GridView q = p;
-        q.slab = const_cast<float *>(plo);
-        lc = voxel_at(q, x, y, z);
-        ln[0] = x + 1 < p.sx ? voxel_at(q, x + 1, y, z) : lc;
-        ln[1] = y + 1 < p.sy ? voxel_at(q, x, y + 1, z) : lc;
-        ln[2] = z + 1 < p.sz ? voxel_at(q, x, y, z + 1) : lc;
+    if constexpr (LO) {  // plo has p's slot layout
+        lc = plo[sp + vid];
+        const float qx = plo[hx ? cx : sp + vid], qy = plo[hy ? cy : sp + vid], qz = plo[hz ? cz : sp + vid];
+        ln[0] = hx ? qx : lc;
+        ln[1] = hy ? qy : lc;
+        ln[2] = hz ? qz : lc;
     }
     const GridView ch[3] = {u, v, w};
+    int slots[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) slots[a] = ch[a].b2s[bid];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int slot = ch[a].b2s[bid];
-        if (slot < 2) continue;
-        float *f = ch[a].slab + ((int64_t)slot << 12) + vid;
+        if (slots[a] < 2) continue;
+        float *f = ch[a].slab + ((int64_t)slots[a] << 12) + vid;
         float g = pn[a] - pc;
         if constexpr (LO) g = g + (ln[a] - lc);
         const float t = scale * g;
