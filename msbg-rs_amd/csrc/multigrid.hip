@@ -235,6 +235,12 @@ __device__ __forceinline__ float blk_interp(const float *__restrict__ xc, const 
                                             const PBlock &Bk, const Dom &d, i64 k, int cx, int cy, int cz) {
     constexpr int C3 = C * C * C, hc = 16 / C;
     const int c[3] = {cx, cy, cz}, org[3] = {Bk.ox, Bk.oy, Bk.oz}, dom[3] = {d.sx, d.sy, d.sz};
+    // Loads in three batches (the -x / -y / -z neighbour ids, the 8 partners'
+    // block ids, the 8 values), each load at a valid address whether or not
+    // its value is used; the sum runs over the partners in the same order.
+    int32_t tm[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) tm[a] = T[nb27_at(a == 0 ? -1 : 0, a == 1 ? -1 : 0, a == 2 ? -1 : 0)];
     int p[3], dd[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -242,20 +248,15 @@ __device__ __forceinline__ float blk_interp(const float *__restrict__ xc, const 
         dd[a] = (c[a] & 1) ? 1 : -1;
         // a partner beyond the domain is the cell itself (clamp)
         if (dd[a] > 0 && org[a] + (p[a] + 1) * hc >= dom[a]) dd[a] = 0;
-        if (dd[a] < 0 && p[a] == 0) {
-            const int o[3] = {a == 0 ? -1 : 0, a == 1 ? -1 : 0, a == 2 ? -1 : 0};
-            if (T[nb27_at(o[0], o[1], o[2])] == -2) dd[a] = 0;
-        }
+        if (dd[a] < 0 && p[a] == 0 && tm[a] == -2) dd[a] = 0;
     }
-    float v = 0.0f;
+    int qi[8], tix[8];
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
         int q[3], o[3];
-        float wt = 1.0f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int on = (kk >> a) & 1;
-            wt *= on ? 0.25f : 0.75f;
             q[a] = p[a] + (on ? dd[a] : 0);
             o[a] = 0;
             if (q[a] < 0) {
@@ -266,8 +267,26 @@ __device__ __forceinline__ float blk_interp(const float *__restrict__ xc, const 
                 o[a] = 1;
             }
         }
-        const i64 nbk = (o[0] | o[1] | o[2]) ? (i64)T[nb27_at(o[0], o[1], o[2])] : k;
-        if (nbk >= 0) v += wt * xc[nbk * C3 + q[0] + C * (q[1] + C * q[2])];
+        qi[kk] = q[0] + C * (q[1] + C * q[2]);
+        tix[kk] = nb27_at(o[0], o[1], o[2]);  // 13: the block itself
+    }
+    int32_t tv[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) tv[kk] = T[tix[kk]];
+    i64 nbk[8];
+    float xv[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        nbk[kk] = tix[kk] == 13 ? k : (i64)tv[kk];
+        xv[kk] = xc[(nbk[kk] >= 0 ? nbk[kk] : k) * C3 + qi[kk]];
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+        float wt = 1.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) wt *= ((kk >> a) & 1) ? 0.25f : 0.75f;
+        if (nbk[kk] >= 0) v += wt * xv[kk];
     }
     return v;
 }
