@@ -415,12 +415,21 @@ __global__ __launch_bounds__(256) void prolong0_kernel(const PBlock *__restrict_
 __device__ __forceinline__ float list_residual(const float *__restrict__ x, const float *__restrict__ b,
                                                const int32_t *__restrict__ nb, const float *__restrict__ cf,
                                                const float *__restrict__ diag, i64 i) {
-    float s = 0.0f;
+    // neighbour indices and coefficients, then the neighbour values, each in
+    // one batch (an absent neighbour reads the cell itself, unused)
+    int32_t q[6];
+    float c[6], xv[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const int32_t q = nb[i * 6 + f];
-        if (q >= 0) s += cf[i * 6 + f] * x[q];
+        q[f] = nb[i * 6 + f];
+        c[f] = cf[i * 6 + f];
     }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) xv[f] = x[q[f] >= 0 ? (i64)q[f] : i];
+    float s = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (q[f] >= 0) s += c[f] * xv[f];
     return b[i] + s - diag[i] * x[i];
 }
 
@@ -449,12 +458,18 @@ __global__ __launch_bounds__(kT) void restrict_list_kernel(i64 nc, const int32_t
                                                            float *bc) {
     const i64 i = (i64)blockIdx.x * kT + threadIdx.x;
     if (i >= nc) return;
+    int32_t c[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) c[o] = ch[i * 8 + o];
+    // every child's residual is evaluated (an absent child at cell 0, unused),
+    // so no branch separates one child's loads from the next one's
+    float r[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) r[o] = list_residual(x, b, nb, cf, diag, c[o] >= 0 ? c[o] : 0);
     float sum = 0.0f;
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-        const int32_t c = ch[i * 8 + o];
-        if (c >= 0) sum += list_residual(x, b, nb, cf, diag, c);
-    }
+    for (int o = 0; o < 8; ++o)
+        if (c[o] >= 0) sum += r[o];
     bc[i] = 0.5f * sum;
 }
 
@@ -462,12 +477,17 @@ __global__ __launch_bounds__(kT) void prolong_list_kernel(i64 n, const int32_t *
                                                           const float *__restrict__ xc, float *x) {
     const i64 i = (i64)blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
+    int32_t q[8];
+    float xv[8];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) q[kk] = pn[i * 8 + kk];
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) xv[kk] = xc[q[kk] >= 0 ? q[kk] : 0];  // an absent partner reads entry 0, unused
     float v = 0.0f;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
-        const int32_t q = pn[i * 8 + kk];
         const float wt = ((kk & 1) ? 0.25f : 0.75f) * ((kk & 2) ? 0.25f : 0.75f) * ((kk & 4) ? 0.25f : 0.75f);
-        if (q >= 0) v += wt * xc[q];
+        if (q[kk] >= 0) v += wt * xv[kk];
     }
     x[i] += v;
 }
