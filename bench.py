@@ -394,7 +394,7 @@ def solve_scale_leg(pkg, lib, S, reps):
     return out
 
 
-def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
+def flip_leg(pkg, lib, S, n, max_sweeps, reps=1, variants=None):
     """One FLIP-style step (configuration C5's particles: dam-break box +
     droplets, S^3, n particles): Morton sort, MAC P2G, divergence, band
     pressure solve (target 1e-6, at most max_sweeps sweeps), projection,
@@ -412,7 +412,8 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
     cap = int(active_blocks(S, S, S, dp, 3.0, lib=lib).size * 1.1) + 64  # above the P2G footprint (r_p = 2)
     fs = FlipStep(S, S, S, capacity_blocks=cap, lib=lib)
     out = {"domain": S, "particles": n, "capacity_blocks": cap}
-    for name, storage, method, comp in FLIP_VARIANTS:
+    chosen = [v for v in FLIP_VARIANTS if variants is None or v[0] in variants]
+    for name, storage, method, comp in chosen:
         args = solve_args(method)
         if method != "mg":
             args.update(sweeps_per_check=10, max_sweeps=max_sweeps)
@@ -434,8 +435,9 @@ def flip_leg(pkg, lib, S, n, max_sweeps, reps=1):
                      "particles_per_s_transfers": n / ((best["ms"]["sort"] + best["ms"]["p2g"] + best["ms"]["g2p"]) * 1e-3)}
         log("flip %s: %.1f ms total, %d sweeps, rel %.2e" % (name, best["ms"]["total"], sv["sweeps"],
                                                              sv["rel_residual"]))
-    for k in [v[0] for v in FLIP_VARIANTS[1:]]:
-        out[k + "_vs_f32_total"] = out[k]["ms"]["total"] / out["f32_plain"]["ms"]["total"]
+    for k in [v[0] for v in chosen if v[0] != "f32_plain"]:
+        if "f32_plain" in out:
+            out[k + "_vs_f32_total"] = out[k]["ms"]["total"] / out["f32_plain"]["ms"]["total"]
     out["note"] = ("single resolution level at S^3; the pressure band is the MAC blocks (those the particles' P2G "
                    "footprint touches) whose -x/-y/-z neighbours are MAC blocks, p = 0 outside it; div_before / "
                    "div_after are L2 norms over the band's cells; an FP32 pressure stalls near 2.3e-7 ||p||/||b||, "
@@ -639,6 +641,8 @@ def main():
                     help="domain edge of the FLIP-step leg (dam-break box + droplets; 0 = skip)")
     ap.add_argument("--flip-particles", type=int, default=64 << 20)
     ap.add_argument("--flip-sweeps", type=int, default=6000, help="sweep cap of the FLIP pressure solve")
+    ap.add_argument("--flip-variants", default="", help="comma-separated subset of the FLIP step variants "
+                    "(f32_plain, f16_refine, f32_mg, f16_mg, f16_mg_pair); all by default")
     ap.add_argument("--flip-fp64", type=int, default=256, metavar="S",
                     help="domain edge of the FLIP step compared with the FP64 CG oracle (0 = skip)")
     ap.add_argument("--skip-solves", action="store_true",
@@ -792,7 +796,8 @@ def main():
         out["surface"] = surface_leg(pkg, lib, args.surface, 8171, 64)
     if args.flip > 0:
         log("FLIP-step leg ...")
-        out["flip"] = flip_leg(pkg, lib, args.flip, args.flip_particles, args.flip_sweeps)
+        out["flip"] = flip_leg(pkg, lib, args.flip, args.flip_particles, args.flip_sweeps,
+                               variants=[v for v in args.flip_variants.split(",") if v] or None)
     from msbg_rs_amd import scenes
     if args.skip_solves:
         print(json.dumps(out), flush=True)
