@@ -270,6 +270,9 @@ __global__ __launch_bounds__(256) void reduce_norms_kernel(const double2 *__rest
                                                            double *norm, float *scale) {
     __shared__ double s_a[256], s_b[256];
     double a = 0.0, b = 0.0;
+    // unrolled so a thread's next partials are in flight while it adds (the
+    // same terms in the same order)
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += 256) {
         a += partial[i].x;
         b += partial[i].y;
