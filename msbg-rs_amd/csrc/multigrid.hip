@@ -314,11 +314,16 @@ __global__ __launch_bounds__(kT) void restrict0_kernel(i64 n_active, const PBloc
     const PBlock B = blocks[i >> 9];
     const int v = (int)(i & 511), ux = v & 7, uy = (v >> 3) & 7, uz = v >> 6;
     const i64 base = ((i64)B.slot << 12);
+    // the 8 children as 4 x-pairs, all loaded before the sum (same terms, same order)
+    float2 pr[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+        pr[h] = *reinterpret_cast<const float2 *>(r + base + 2 * ux + 16 * (2 * uy + (h & 1)) + 256 * (2 * uz + (h >> 1)));
     float sum = 0.0f;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
         const int fx = 2 * ux + (o & 1), fy = 2 * uy + ((o >> 1) & 1), fz = 2 * uz + (o >> 2);
-        if (live(B, d, 1, fx, fy, fz)) sum += r[base + fx + 16 * fy + 256 * fz];
+        if (live(B, d, 1, fx, fy, fz)) sum += (o & 1) ? pr[o >> 1].y : pr[o >> 1].x;
     }
     b1[i] = 0.5f * sum;
 }
